@@ -1,0 +1,174 @@
+/*
+ * lbmx.h -- native, device-resident C ABI of the MI355X lattice-Boltzmann library.
+ *
+ * The reference library (include/lbm_core.h, lbm_bc.h) is a set of host-array
+ * functions called once per time step.  This header is the thin C layer under
+ * that drop-in API: a lattice lives in GPU memory between calls, its boundary
+ * table is compiled once, and a time step is a short sequence of gfx950 kernels
+ * on the lattice's own HIP stream.  Plain pointers and sizes only; no HIP or
+ * torch types appear here, so C, ctypes or any other FFI can bind it.
+ *
+ * Every function returns LBMX_OK (0) or a negative error code; lbmx_last_error()
+ * describes the last failure of the calling thread.
+ *
+ * Array conventions (the reference's, lbm_core.h:10-31):
+ *   populations  double f[Ny][Nx][9]  (array of structures, k fastest)
+ *   fields       double a[Ny][Nx]     (row-major)
+ *   solid mask   int    m[Ny][Nx]     (nonzero = solid)
+ * D2Q9 ordering cx={0,1,0,-1,0,1,-1,-1,1}, cy={0,0,1,0,-1,1,1,-1,-1},
+ * w={4/9,1/9 x4,1/36 x4}, opp={0,3,4,1,2,7,8,5,6} (cases/nit/nit.c:195-202).
+ */
+#ifndef LBMX_H
+#define LBMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBMX_OK 0
+#define LBMX_ERR_ARG (-1)     /* invalid argument or unsupported configuration */
+#define LBMX_ERR_HIP (-2)     /* a HIP runtime call failed */
+#define LBMX_ERR_STATE (-3)   /* call not valid in the lattice's current state */
+#define LBMX_ERR_NOMEM (-4)   /* device allocation failed */
+
+/* Storage / arithmetic of the populations. */
+#define LBMX_F64 0            /* FP64 populations, FP64 collision (reference arithmetic) */
+#define LBMX_F32_SHIFTED 1    /* FP32 populations stored as f_k - w_k*rho0, FP32 collision */
+
+/* Boundary-condition codes: one per BCFunction of include/lbm_bc.h. */
+#define LBMX_BC_CONVECTIVE_INLET 1          /* src/bc/convective_inlet.c */
+#define LBMX_BC_CONVECTIVE_OUTLET 2         /* src/bc/convective_outlet.c */
+#define LBMX_BC_PRESSURE_INLET 3            /* src/bc/pressure_inlet.c */
+#define LBMX_BC_PRESSURE_OUTLET 4           /* src/bc/pressure_outlet.c */
+#define LBMX_BC_PERIODIC_X 5                /* src/bc/periodic_x.c */
+#define LBMX_BC_PERIODIC_Y 6                /* src/bc/periodic_y.c */
+#define LBMX_BC_VELOCITY_INLET 7            /* src/bc/velocity_inlet.c */
+#define LBMX_BC_VELOCITY_OUTLET 8           /* src/bc/velocity_outlet.c */
+#define LBMX_BC_VELOCITY_TOP 9              /* src/bc/velocity_top.c */
+#define LBMX_BC_VELOCITY_BOTTOM 10          /* src/bc/velocity_bottom.c */
+#define LBMX_BC_VELOCITY_OUTLET_REG 11      /* src/bc/velocity_outlet_regularized.c */
+#define LBMX_BC_VELOCITY_INLET_REG 12       /* src/bc/velocity_inlet_regularized.c */
+#define LBMX_BC_VELOCITY_TOP_REG 13         /* src/bc/velocity_top_regularized.c */
+#define LBMX_BC_VELOCITY_BOTTOM_REG 14      /* src/bc/velocity_bottom_regularized.c */
+#define LBMX_BC_PRESSURE_OUTLET_REG 15      /* src/bc/pressure_outlet_regularized.c */
+#define LBMX_BC_PRESSURE_INLET_REG 16       /* src/bc/pressure_inlet_regularized.c */
+#define LBMX_BC_PRESSURE_TOP_REG 17         /* src/bc/pressure_top_regularized.c */
+#define LBMX_BC_PRESSURE_BOTTOM_REG 18      /* src/bc/pressure_bottom_regularized.c */
+#define LBMX_BC_CONVECTIVE_TOP 19           /* src/bc/convective_top.c (not declared in lbm_bc.h) */
+#define LBMX_BC_CONVECTIVE_BOTTOM 20        /* src/bc/convective_bottom.c (not declared in lbm_bc.h) */
+#define LBMX_BC_MAX 20
+
+/* Lattice options (lbmx_config.flags). */
+#define LBMX_EXACT_STATE 1u   /* keep the reference's full two-buffer state every step (both
+                                 buffers downloadable); costs one extra population write */
+
+typedef struct lbmx_lattice lbmx_lattice;
+
+typedef struct {
+    int Nx, Ny;
+    int precision;        /* LBMX_F64 or LBMX_F32_SHIFTED */
+    int per_x, per_y;     /* periodic wrap in streaming (lbm_core.h:23-24) */
+    int use_ibb;          /* interpolated bounce-back at solid links (lbm_core.h:30) */
+    double Fx, Fy;        /* Guo body force (lbm_core.h:25-26) */
+    double rho0;          /* shift density of LBMX_F32_SHIFTED storage (usually 1) */
+    unsigned flags;       /* LBMX_EXACT_STATE */
+} lbmx_config;
+
+const char *lbmx_last_error(void);
+/* Name of the device the library runs on (for logs), or "" when none. */
+const char *lbmx_device_name(void);
+
+/* Replaces main_lbm's per-call arrays (lbm_core.h:6-31) by a resident lattice. */
+int lbmx_create(const lbmx_config *cfg, lbmx_lattice **out);
+void lbmx_destroy(lbmx_lattice *lat);
+
+/* Geometry: solid mask [Ny][Nx] (required), omega_eff [Ny][Nx] (required),
+ * phi [Ny][Nx] signed distance (required when use_ibb).  Copied; the link list
+ * and node classes are compiled here.  Replaces the solid_mask/omega_eff/phi
+ * arguments of main_lbm (lbm_core.h:19-20, 29). */
+int lbmx_set_geometry(lbmx_lattice *lat, const int *solid_mask, const double *omega,
+                      const double *phi);
+
+/* Boundary table in application order (main_lbm.c:49-58): types[b] is an
+ * LBMX_BC_* code, index[b] the column (inlet/outlet) or row (top/bottom). */
+int lbmx_set_boundaries(lbmx_lattice *lat, const int *types, const int *index, int nbc);
+
+/* Values of boundary b for the next step: val1/val2 are host arrays of the edge
+ * length (Ny for inlet/outlet, Nx for top/bottom) or NULL when the boundary does
+ * not read them (Boundary.val1/val2, lbm_types.h:22-27). */
+int lbmx_set_bc_values(lbmx_lattice *lat, int b, const double *val1, const double *val2);
+
+/* Starts a run from host state in the reference layout: f is what the caller's
+ * `f` holds before main_lbm, f_new what its `f_new` holds (its entries that the
+ * step does not overwrite survive, as in streaming.c:36-40), and rho/u/v the
+ * moment arrays the first collision reads (collision.c:24-32).  f_new may be
+ * NULL (then it is a copy of f). */
+int lbmx_upload(lbmx_lattice *lat, const double *f, const double *f_new, const double *rho,
+                const double *u, const double *v);
+
+/* Starts a run from a Taylor-Green vortex generated on the device (no host
+ * arrays): u = u0 (sin kx x cos ky y, -(kx/ky) cos kx x sin ky y),
+ * rho = 1 - (3 u0^2/4)(cos 2kx x + (kx/ky)^2 cos 2ky y), both buffers at
+ * equilibrium, moment arrays = (rho, u, v).  For benchmarks and large runs
+ * (the configuration of cases/taylorgreen/taylorgreen.c with periodic axes). */
+int lbmx_init_taylor_green(lbmx_lattice *lat, double u0, double kx, double ky);
+
+/* Advances nsteps time steps; each is main_lbm followed by the driver's f/f_new
+ * swap.  Asynchronous on the lattice's stream. */
+int lbmx_step(lbmx_lattice *lat, int nsteps);
+
+/* Momentum-exchange force of the last step, as main_lbm returns it through
+ * (L, D) (main_lbm.c:43): D = x-component, L = y-component. */
+int lbmx_forces(lbmx_lattice *lat, double *L, double *D);
+
+/* Moments of the current state exactly as compute_macroscopic_fields leaves
+ * them after the last step (NaN on solid nodes).  Any pointer may be NULL. */
+int lbmx_moments(lbmx_lattice *lat, double *rho, double *u, double *v);
+
+/* Current populations: f receives what the driver's `f` points to after the
+ * last step and swap.  f_new (may be NULL) receives the other buffer; it is the
+ * reference's content only for lattices created with LBMX_EXACT_STATE. */
+int lbmx_download(lbmx_lattice *lat, double *f, double *f_new);
+
+/* Blocks until the lattice's queued work is done. */
+int lbmx_synchronize(lbmx_lattice *lat);
+
+/* Steps run since lbmx_upload, and device bytes held by the lattice. */
+int64_t lbmx_steps_done(const lbmx_lattice *lat);
+int64_t lbmx_device_bytes(const lbmx_lattice *lat);
+
+/* Mei consistent initialization on the device (mei_initialization.c:7-142):
+ * same arguments and in/out contract as the reference function with the
+ * lattice tables implied.  f/f_new are read as the starting buffers and
+ * written back; rho/u/v receive the reference's outputs.  Returns the number
+ * of iterations run (>= 1) or a negative error code. */
+int lbmx_mei_initialization(int Nx, int Ny, double *f, double *f_new, const double *rho_in,
+                            const double *U_in, const double *V_in, double *rho, double *u,
+                            double *v, const double *omega, int per_x, int per_y,
+                            int init_iter_max, double tol_rho, const int *solid_mask);
+
+/* Single-stage entry points behind the reference's stand-alone functions
+ * (lbm_core.h:33-92, lbm_bc.h:4-355).  Each uploads its host arrays, runs the
+ * stage on the device with the reference's semantics and writes its outputs
+ * back.  bc_type is an LBMX_BC_* code. */
+int lbmx_stage_collision(int Nx, int Ny, double *f, const double *rho, const double *u,
+                         const double *v, const double *omega, const int *mask, double Fx,
+                         double Fy);
+int lbmx_stage_streaming(int Nx, int Ny, const double *f, double *f_new, const int *mask,
+                         int per_x, int per_y);
+int lbmx_stage_bounce_back(int Nx, int Ny, const double *f, double *f_new, const int *mask,
+                           int per_x, int per_y, int use_ibb, const double *phi);
+int lbmx_stage_force(int Nx, int Ny, const double *f, const double *f_new, const int *mask,
+                     int per_x, int per_y, double *Fx, double *Fy);
+int lbmx_stage_moments(int Nx, int Ny, const double *f_new, const int *mask, double *rho,
+                       double *u, double *v, double Fx, double Fy);
+int lbmx_stage_bc(int Nx, int Ny, int bc_type, int index, const double *val1,
+                  const double *val2, const double *f, double *f_new, const double *rho,
+                  const double *u, const double *v, const int *mask);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,749 @@
+// lattice.hip -- the resident lattice behind include/lbmx.h.
+//
+// Host-side work here is set-up only: geometry compilation (node classes, the
+// solid-link list with its wall fractions q), boundary-table compilation and
+// buffer management.  Every per-step operation runs on the device.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "lbmx.h"
+#include "lattice_impl.h"
+#include "step_kernels.h"
+
+using namespace lbmx;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+int lbmx_fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+extern "C" const char *lbmx_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char *lbmx_device_name(void)
+{
+    static std::string name;
+    if (name.empty()) {
+        int dev = 0;
+        hipDeviceProp_t p;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+            name = std::string(p.name) + " (" + p.gcnArchName + ")";
+    }
+    return name.c_str();
+}
+
+int lbmx_device_info(int *ncu)
+{
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        HIPCHK(hipGetDeviceProperties(&p, dev));
+        cached = p.multiProcessorCount;
+    }
+    *ncu = cached;
+    return LBMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// boundary-table helpers
+// ---------------------------------------------------------------------------
+int lbmx_bc_on_column(int t)
+{
+    return t == LBMX_BC_CONVECTIVE_INLET || t == LBMX_BC_CONVECTIVE_OUTLET ||
+           t == LBMX_BC_PRESSURE_INLET || t == LBMX_BC_PRESSURE_OUTLET ||
+           t == LBMX_BC_VELOCITY_INLET || t == LBMX_BC_VELOCITY_OUTLET ||
+           t == LBMX_BC_VELOCITY_OUTLET_REG || t == LBMX_BC_VELOCITY_INLET_REG ||
+           t == LBMX_BC_PRESSURE_OUTLET_REG || t == LBMX_BC_PRESSURE_INLET_REG;
+}
+
+// 0 = no values, 1 = val1 only, 2 = val1 and val2 (which the reference reads)
+int lbmx_bc_nvals(int t)
+{
+    switch (t) {
+    case LBMX_BC_VELOCITY_INLET: case LBMX_BC_VELOCITY_OUTLET: case LBMX_BC_VELOCITY_TOP:
+    case LBMX_BC_VELOCITY_BOTTOM: case LBMX_BC_VELOCITY_OUTLET_REG:
+    case LBMX_BC_VELOCITY_INLET_REG: case LBMX_BC_VELOCITY_TOP_REG:
+    case LBMX_BC_VELOCITY_BOTTOM_REG:
+        return 2;
+    case LBMX_BC_PRESSURE_INLET: case LBMX_BC_PRESSURE_OUTLET:
+    case LBMX_BC_PRESSURE_OUTLET_REG: case LBMX_BC_PRESSURE_INLET_REG:
+    case LBMX_BC_PRESSURE_TOP_REG: case LBMX_BC_PRESSURE_BOTTOM_REG:
+        return 1;
+    default:
+        return 0;
+    }
+}
+
+// Range check of index so that every neighbour a BC reads lies in the domain.
+int lbmx_bc_check(int t, int index, int Nx, int Ny)
+{
+    if (t < 1 || t > LBMX_BC_MAX) return lbmx_fail(LBMX_ERR_ARG, "unknown boundary type %d", t);
+    if (t == LBMX_BC_PERIODIC_X || t == LBMX_BC_PERIODIC_Y) return LBMX_OK;
+    const int L = lbmx_bc_on_column(t) ? Nx : Ny;
+    int lo = 0, hi = L - 1;
+    switch (t) {
+    case LBMX_BC_CONVECTIVE_INLET: case LBMX_BC_PRESSURE_INLET:
+    case LBMX_BC_VELOCITY_INLET_REG: case LBMX_BC_PRESSURE_INLET_REG:
+    case LBMX_BC_VELOCITY_BOTTOM_REG: case LBMX_BC_PRESSURE_BOTTOM_REG:
+    case LBMX_BC_CONVECTIVE_BOTTOM:
+        hi = L - 2; break;
+    case LBMX_BC_CONVECTIVE_OUTLET: case LBMX_BC_PRESSURE_OUTLET:
+    case LBMX_BC_VELOCITY_OUTLET_REG: case LBMX_BC_PRESSURE_OUTLET_REG:
+    case LBMX_BC_VELOCITY_TOP_REG: case LBMX_BC_PRESSURE_TOP_REG:
+    case LBMX_BC_CONVECTIVE_TOP:
+        lo = 1; break;
+    default: break;
+    }
+    if (index < lo || index > hi)
+        return lbmx_fail(LBMX_ERR_ARG, "boundary type %d: index %d outside [%d, %d]", t, index,
+                         lo, hi);
+    return LBMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// lattice lifecycle
+// ---------------------------------------------------------------------------
+static size_t esize_of(int prec) { return prec == LBMX_F64 ? 8 : 4; }
+
+extern "C" int lbmx_create(const lbmx_config *cfg, lbmx_lattice **out)
+{
+    if (!cfg || !out) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->Nx < 3 || cfg->Ny < 3)
+        return lbmx_fail(LBMX_ERR_ARG, "lattice %dx%d: both sizes must be >= 3", cfg->Nx,
+                         cfg->Ny);
+    if (cfg->precision != LBMX_F64 && cfg->precision != LBMX_F32_SHIFTED)
+        return lbmx_fail(LBMX_ERR_ARG, "unknown precision %d", cfg->precision);
+    if ((long long)cfg->Nx * cfg->Ny > (1LL << 31) - (1LL << 24))
+        return lbmx_fail(LBMX_ERR_ARG, "lattice too large for 32-bit node indexing");
+
+    lbmx_lattice *L = new lbmx_lattice();
+    L->cfg = *cfg;
+    if (L->cfg.precision == LBMX_F32_SHIFTED && L->cfg.rho0 == 0.0) L->cfg.rho0 = 1.0;
+    if (L->cfg.precision == LBMX_F64) L->cfg.rho0 = 0.0;
+    L->Nx = cfg->Nx;
+    L->Ny = cfg->Ny;
+    L->esize = esize_of(cfg->precision);
+    L->pitch = (cfg->Nx + 63) / 64 * 64;
+    // planes are separated by a small gap so that the nine plane bases of a
+    // large lattice do not share the same high-order address bits
+    L->plane = (size_t)L->Ny * L->pitch + 64 * 3;
+    int rc = lbmx_device_info(&L->ncu);
+    if (rc) { delete L; return rc; }
+    L->nxcd = 8;
+
+    hipError_t e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete L;
+        return lbmx_fail(LBMX_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    const size_t pop_bytes = (size_t)kQ * L->plane * L->esize;
+    const size_t fld_bytes = (size_t)L->Ny * L->pitch * sizeof(double);
+    if (hipMalloc(&L->pop[0], pop_bytes) != hipSuccess ||
+        hipMalloc(&L->pop[1], pop_bytes) != hipSuccess ||
+        hipMalloc((void **)&L->R, fld_bytes) != hipSuccess ||
+        hipMalloc((void **)&L->U, fld_bytes) != hipSuccess ||
+        hipMalloc((void **)&L->V, fld_bytes) != hipSuccess ||
+        hipMalloc((void **)&L->force_dev, 2 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void **)&L->force_host, 2 * sizeof(double), hipHostMallocDefault) !=
+            hipSuccess) {
+        lbmx_destroy(L);
+        return lbmx_fail(LBMX_ERR_NOMEM, "device allocation of %zu bytes failed",
+                         2 * pop_bytes + 3 * fld_bytes);
+    }
+    L->bytes = 2 * pop_bytes + 3 * fld_bytes;
+    // population gaps and pitch padding are never read as data; clear once so
+    // every byte of the buffers is defined
+    hipMemsetAsync(L->pop[0], 0, pop_bytes, L->stream);
+    hipMemsetAsync(L->pop[1], 0, pop_bytes, L->stream);
+    hipMemsetAsync(L->force_dev, 0, 2 * sizeof(double), L->stream);
+    // default geometry: all fluid, omega = 1, no boundaries
+    L->geom = 0;
+    L->om_uniform = 1.0;
+    L->bulk_grid = L->ncu * 8;
+    L->bulk_grid = (L->bulk_grid + 7) / 8 * 8;
+    hipStreamSynchronize(L->stream);
+    *out = L;
+    return LBMX_OK;
+}
+
+extern "C" void lbmx_destroy(lbmx_lattice *L)
+{
+    if (!L) return;
+    if (L->stream) hipStreamSynchronize(L->stream);
+    hipFree(L->pop[0]);
+    hipFree(L->pop[1]);
+    hipFree(L->R);
+    hipFree(L->U);
+    hipFree(L->V);
+    hipFree(L->cls);
+    hipFree(L->lut);
+    hipFree(L->om_arr);
+    hipFree(L->links);
+    hipFree(L->partials);
+    hipFree(L->force_dev);
+    hipFree(L->bc_dev);
+    for (double *p : L->val1) hipFree(p);
+    for (double *p : L->val2) hipFree(p);
+    hipFree(L->staging);
+    if (L->force_host) hipHostFree(L->force_host);
+    if (L->stream) hipStreamDestroy(L->stream);
+    delete L;
+}
+
+// ---------------------------------------------------------------------------
+// geometry compilation
+// ---------------------------------------------------------------------------
+static bool wrap_or_reject(int &c, int n, int periodic)
+{
+    if (periodic) {
+        if (c < 0) c += n;
+        else if (c >= n) c -= n;
+        return true;
+    }
+    return c >= 0 && c < n;
+}
+
+extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double *omega,
+                                 const double *phi)
+{
+    if (!L || !mask || !omega) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (L->cfg.use_ibb && !phi) return lbmx_fail(LBMX_ERR_ARG, "use_ibb requires phi");
+    const int Nx = L->Nx, Ny = L->Ny, pitch = L->pitch;
+    const size_t nn = (size_t)Nx * Ny;
+    static const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    static const int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+
+    // node classes: distinct fluid omegas (bitwise) -> class 1..255, solid -> 0
+    size_t nsolid = 0;
+    std::map<uint64_t, int> cls_of;
+    std::vector<double> lut(256, 0.0);
+    bool fits = true;
+    uint64_t last_bits = 0;
+    bool have_last = false;
+    for (size_t n = 0; n < nn; n++) {
+        if (mask[n]) { nsolid++; continue; }
+        uint64_t bits;
+        memcpy(&bits, &omega[n], 8);
+        if (have_last && bits == last_bits) continue;
+        last_bits = bits;
+        have_last = true;
+        if (fits && !cls_of.count(bits)) {
+            if (cls_of.size() >= 255) fits = false;
+            else {
+                int c = (int)cls_of.size() + 1;
+                cls_of[bits] = c;
+                lut[c] = omega[n];
+            }
+        }
+    }
+    hipFree(L->cls); L->cls = nullptr;
+    hipFree(L->lut); L->lut = nullptr;
+    hipFree(L->om_arr); L->om_arr = nullptr;
+    L->nsolid = (long long)nsolid;
+    if (nsolid == 0 && cls_of.size() == 1) {
+        L->geom = 0;
+        L->om_uniform = lut[1];
+    } else {
+        L->geom = fits ? 1 : 2;
+        std::vector<uint8_t> cls((size_t)Ny * pitch, 0);
+        uint64_t cbits = 0;
+        uint8_t ccls = 0;
+        bool have_c = false;
+        for (int j = 0; j < Ny; j++)
+            for (int i = 0; i < Nx; i++) {
+                size_t n = (size_t)j * Nx + i;
+                uint8_t c = 0;
+                if (!mask[n]) {
+                    if (fits) {
+                        uint64_t bits;
+                        memcpy(&bits, &omega[n], 8);
+                        if (!have_c || bits != cbits) {
+                            cbits = bits;
+                            ccls = (uint8_t)cls_of[bits];
+                            have_c = true;
+                        }
+                        c = ccls;
+                    } else {
+                        c = 1;
+                    }
+                }
+                cls[(size_t)j * pitch + i] = c;
+            }
+        HIPCHK(hipMalloc((void **)&L->cls, cls.size()));
+        HIPCHK(hipMemcpy(L->cls, cls.data(), cls.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc((void **)&L->lut, 256 * sizeof(double)));
+        HIPCHK(hipMemcpy(L->lut, lut.data(), 256 * sizeof(double), hipMemcpyHostToDevice));
+        if (!fits) {
+            HIPCHK(hipMalloc((void **)&L->om_arr, (size_t)Ny * pitch * sizeof(double)));
+            HIPCHK(hipMemcpy2D(L->om_arr, pitch * sizeof(double), omega, Nx * sizeof(double),
+                               Nx * sizeof(double), Ny, hipMemcpyHostToDevice));
+        }
+    }
+
+    // fluid -> solid links, row-major, k ascending (compute_force_mem.c:26-58);
+    // q and the behind node as interpolated_bounce_back.c:51-89 derives them
+    std::vector<Link> links;
+    const double eps = 1e-12;
+    const int px = L->cfg.per_x, py = L->cfg.per_y;
+    if (nsolid) {
+        for (int j = 0; j < Ny; j++)
+            for (int i = 0; i < Nx; i++) {
+                if (mask[(size_t)j * Nx + i]) continue;
+                for (int k = 1; k < 9; k++) {
+                    int in = i + CX[k], jn = j + CY[k];
+                    if (!wrap_or_reject(in, Nx, px) || !wrap_or_reject(jn, Ny, py)) continue;
+                    if (!mask[(size_t)jn * Nx + in]) continue;
+                    Link l;
+                    l.n = (long long)j * pitch + i;
+                    l.k = k;
+                    l.q = 0.5;
+                    l.nb = -1;
+                    l.bsolid = 0;
+                    if (L->cfg.use_ibb) {
+                        double den = phi[(size_t)j * Nx + i] - phi[(size_t)jn * Nx + in];
+                        if (fabs(den) < eps) den = (den >= 0.0) ? eps : -eps;
+                        double q = phi[(size_t)j * Nx + i] / den;
+                        if (q < eps) q = eps;
+                        if (q > 1.0 - eps) q = 1.0 - eps;
+                        l.q = q;
+                        int ib = i - CX[k], jb = j - CY[k];
+                        if (wrap_or_reject(ib, Nx, px) && wrap_or_reject(jb, Ny, py)) {
+                            l.nb = (long long)jb * pitch + ib;
+                            l.bsolid = mask[(size_t)jb * Nx + ib] ? 1 : 0;
+                        }
+                    }
+                    links.push_back(l);
+                }
+            }
+    }
+    hipFree(L->links); L->links = nullptr;
+    hipFree(L->partials); L->partials = nullptr;
+    L->nlinks = (int)links.size();
+    L->npartials = (L->nlinks + kLinkThreads - 1) / kLinkThreads;
+    if (L->nlinks) {
+        HIPCHK(hipMalloc((void **)&L->links, links.size() * sizeof(Link)));
+        HIPCHK(hipMemcpy(L->links, links.data(), links.size() * sizeof(Link),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc((void **)&L->partials, 2 * sizeof(double) * L->npartials));
+    }
+    HIPCHK(hipMemsetAsync(L->force_dev, 0, 2 * sizeof(double), L->stream));
+
+    // stale-edge enumeration (k_edges)
+    L->edge_rows = py ? 0 : 2 * Nx;
+    L->edge_cols = px ? 0 : (py ? 2 * Ny : 2 * (Ny - 2));
+    L->has_geometry = 1;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return LBMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// boundary table
+// ---------------------------------------------------------------------------
+extern "C" int lbmx_set_boundaries(lbmx_lattice *L, const int *types, const int *index, int nbc)
+{
+    if (!L || nbc < 0 || (nbc > 0 && (!types || !index)))
+        return lbmx_fail(LBMX_ERR_ARG, "bad boundary table");
+    for (int b = 0; b < nbc; b++) {
+        int rc = lbmx_bc_check(types[b], index[b], L->Nx, L->Ny);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    for (double *p : L->val1) hipFree(p);
+    for (double *p : L->val2) hipFree(p);
+    L->val1.assign(nbc, nullptr);
+    L->val2.assign(nbc, nullptr);
+    L->types.assign(types, types + nbc);
+    L->index.assign(index, index + nbc);
+    L->vals_set.assign(nbc, 0);
+
+    // Leading periodic entries whose axis already wraps in streaming re-write
+    // exactly what streaming wrote (periodic_x.c vs streaming.c:24-34): dropped.
+    std::vector<BcDesc> prog;
+    bool only_redundant_so_far = true;
+    for (int b = 0; b < nbc; b++) {
+        const int t = types[b];
+        const bool redundant = (t == LBMX_BC_PERIODIC_X && L->cfg.per_x) ||
+                               (t == LBMX_BC_PERIODIC_Y && L->cfg.per_y);
+        const int nv = lbmx_bc_nvals(t);
+        const int len = lbmx_bc_on_column(t) ? L->Ny : L->Nx;
+        if (nv >= 1) HIPCHK(hipMalloc((void **)&L->val1[b], len * sizeof(double)));
+        if (nv >= 2) HIPCHK(hipMalloc((void **)&L->val2[b], len * sizeof(double)));
+        if (redundant && only_redundant_so_far) continue;
+        only_redundant_so_far = false;
+        BcDesc d;
+        d.type = t;
+        d.index = index[b];
+        d.v1 = L->val1[b];
+        d.v2 = L->val2[b];
+        prog.push_back(d);
+    }
+    hipFree(L->bc_dev);
+    L->bc_dev = nullptr;
+    L->nprog = (int)prog.size();
+    if (L->nprog) {
+        HIPCHK(hipMalloc((void **)&L->bc_dev, prog.size() * sizeof(BcDesc)));
+        HIPCHK(hipMemcpy(L->bc_dev, prog.data(), prog.size() * sizeof(BcDesc),
+                         hipMemcpyHostToDevice));
+    }
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_set_bc_values(lbmx_lattice *L, int b, const double *v1, const double *v2)
+{
+    if (!L || b < 0 || b >= (int)L->types.size())
+        return lbmx_fail(LBMX_ERR_ARG, "boundary %d out of range", b);
+    const int t = L->types[b];
+    const int nv = lbmx_bc_nvals(t);
+    const int len = lbmx_bc_on_column(t) ? L->Ny : L->Nx;
+    if (nv >= 1) {
+        if (!v1) return lbmx_fail(LBMX_ERR_ARG, "boundary %d (type %d) needs val1", b, t);
+        HIPCHK(hipMemcpyAsync(L->val1[b], v1, len * sizeof(double), hipMemcpyHostToDevice,
+                              L->stream));
+    }
+    if (nv >= 2) {
+        if (!v2) return lbmx_fail(LBMX_ERR_ARG, "boundary %d (type %d) needs val2", b, t);
+        HIPCHK(hipMemcpyAsync(L->val2[b], v2, len * sizeof(double), hipMemcpyHostToDevice,
+                              L->stream));
+    }
+    L->vals_set[b] = 1;
+    return LBMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// state transfer
+// ---------------------------------------------------------------------------
+Geo lbmx_geo(const lbmx_lattice *L)
+{
+    Geo g;
+    g.Nx = L->Nx;
+    g.Ny = L->Ny;
+    g.pitch = L->pitch;
+    g.plane = L->plane;
+    g.geom = L->geom;
+    g.cls = L->cls;
+    g.lut = L->lut;
+    g.om_arr = L->om_arr;
+    g.om = L->om_uniform;
+    g.per_x = L->cfg.per_x;
+    g.per_y = L->cfg.per_y;
+    g.rho0 = L->cfg.rho0;
+    return g;
+}
+
+static int ensure_staging(lbmx_lattice *L, size_t bytes)
+{
+    if (L->staging_bytes >= bytes) return LBMX_OK;
+    hipFree(L->staging);
+    L->staging = nullptr;
+    L->staging_bytes = 0;
+    HIPCHK(hipMalloc((void **)&L->staging, bytes));
+    L->staging_bytes = bytes;
+    return LBMX_OK;
+}
+
+// rows of AoS populations per transfer chunk (<= 64 MiB of staging)
+static int chunk_rows(const lbmx_lattice *L)
+{
+    size_t row = (size_t)L->Nx * kQ * sizeof(double);
+    size_t r = ((size_t)64 << 20) / row;
+    if (r < 1) r = 1;
+    if (r > (size_t)L->Ny) r = L->Ny;
+    return (int)r;
+}
+
+int lbmx_put_pops(lbmx_lattice *L, int which, const double *aos)
+{
+    const int rows = chunk_rows(L);
+    int rc = ensure_staging(L, (size_t)rows * L->Nx * kQ * sizeof(double));
+    if (rc) return rc;
+    Geo g = lbmx_geo(L);
+    for (int y0 = 0; y0 < L->Ny; y0 += rows) {
+        const int r = (y0 + rows <= L->Ny) ? rows : L->Ny - y0;
+        const size_t cnt = (size_t)r * L->Nx;
+        HIPCHK(hipMemcpyAsync(L->staging, aos + (size_t)y0 * L->Nx * kQ, cnt * kQ * 8,
+                              hipMemcpyHostToDevice, L->stream));
+        const unsigned blocks = (unsigned)((cnt + 255) / 256);
+        if (L->esize == 8)
+            hipLaunchKernelGGL(k_aos_to_planes<double>, dim3(blocks), dim3(256), 0, L->stream,
+                               (double *)L->pop[which], L->staging, g, y0, r);
+        else
+            hipLaunchKernelGGL(k_aos_to_planes<float>, dim3(blocks), dim3(256), 0, L->stream,
+                               (float *)L->pop[which], L->staging, g, y0, r);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(L->stream));
+    }
+    return LBMX_OK;
+}
+
+int lbmx_get_pops(lbmx_lattice *L, int which, double *aos)
+{
+    const int rows = chunk_rows(L);
+    int rc = ensure_staging(L, (size_t)rows * L->Nx * kQ * sizeof(double));
+    if (rc) return rc;
+    Geo g = lbmx_geo(L);
+    for (int y0 = 0; y0 < L->Ny; y0 += rows) {
+        const int r = (y0 + rows <= L->Ny) ? rows : L->Ny - y0;
+        const size_t cnt = (size_t)r * L->Nx;
+        const unsigned blocks = (unsigned)((cnt + 255) / 256);
+        if (L->esize == 8)
+            hipLaunchKernelGGL(k_planes_to_aos<double>, dim3(blocks), dim3(256), 0, L->stream,
+                               L->staging, (const double *)L->pop[which], g, y0, r);
+        else
+            hipLaunchKernelGGL(k_planes_to_aos<float>, dim3(blocks), dim3(256), 0, L->stream,
+                               L->staging, (const float *)L->pop[which], g, y0, r);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(aos + (size_t)y0 * L->Nx * kQ, L->staging, cnt * kQ * 8,
+                              hipMemcpyDeviceToHost, L->stream));
+        HIPCHK(hipStreamSynchronize(L->stream));
+    }
+    return LBMX_OK;
+}
+
+int lbmx_put_field(lbmx_lattice *L, double *dev, const double *host)
+{
+    HIPCHK(hipMemcpy2DAsync(dev, L->pitch * sizeof(double), host, L->Nx * sizeof(double),
+                            L->Nx * sizeof(double), L->Ny, hipMemcpyHostToDevice, L->stream));
+    return LBMX_OK;
+}
+
+int lbmx_get_field(lbmx_lattice *L, double *host, const double *dev)
+{
+    HIPCHK(hipMemcpy2DAsync(host, L->Nx * sizeof(double), dev, L->pitch * sizeof(double),
+                            L->Nx * sizeof(double), L->Ny, hipMemcpyDeviceToHost, L->stream));
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_upload(lbmx_lattice *L, const double *f, const double *f_new,
+                           const double *rho, const double *u, const double *v)
+{
+    if (!L || !f || !rho || !u || !v) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!L->has_geometry) return lbmx_fail(LBMX_ERR_STATE, "lbmx_set_geometry first");
+    int rc = lbmx_put_pops(L, L->cur, f);
+    if (!rc) rc = lbmx_put_pops(L, L->cur ^ 1, f_new ? f_new : f);
+    if (!rc) rc = lbmx_put_field(L, L->R, rho);
+    if (!rc) rc = lbmx_put_field(L, L->U, u);
+    if (!rc) rc = lbmx_put_field(L, L->V, v);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    L->lag_arrays = 1;
+    L->has_state = 1;
+    L->steps = 0;
+    L->Fx_lag = L->cfg.Fx;
+    L->Fy_lag = L->cfg.Fy;
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_init_taylor_green(lbmx_lattice *L, double u0, double kx, double ky)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!L->has_geometry) return lbmx_fail(LBMX_ERR_STATE, "lbmx_set_geometry first");
+    if (ky == 0.0) return lbmx_fail(LBMX_ERR_ARG, "ky must be non-zero");
+    const Geo g = lbmx_geo(L);
+    const Fields F{L->R, L->U, L->V};
+    dim3 grid((L->Nx + 255) / 256, L->Ny);
+    if (L->esize == 8)
+        hipLaunchKernelGGL(k_init_tgv<double>, grid, dim3(256), 0, L->stream,
+                           (double *)L->pop[0], (double *)L->pop[1], F, g, u0, kx, ky);
+    else
+        hipLaunchKernelGGL(k_init_tgv<float>, grid, dim3(256), 0, L->stream,
+                           (float *)L->pop[0], (float *)L->pop[1], F, g, u0, kx, ky);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(L->stream));
+    L->lag_arrays = 1;
+    L->has_state = 1;
+    L->steps = 0;
+    L->Fx_lag = L->cfg.Fx;
+    L->Fy_lag = L->cfg.Fy;
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_download(lbmx_lattice *L, double *f, double *f_new)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!L->has_state) return lbmx_fail(LBMX_ERR_STATE, "no state uploaded");
+    int rc = LBMX_OK;
+    if (f) rc = lbmx_get_pops(L, L->cur, f);
+    if (!rc && f_new) rc = lbmx_get_pops(L, L->cur ^ 1, f_new);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// time stepping
+// ---------------------------------------------------------------------------
+StepFlags lbmx_flags(const lbmx_lattice *L)
+{
+    StepFlags s;
+    s.lag_arrays = L->lag_arrays;
+    s.exact = (L->cfg.flags & LBMX_EXACT_STATE) ? 1 : 0;
+    s.Fx = L->cfg.Fx;
+    s.Fy = L->cfg.Fy;
+    s.force = (L->cfg.Fx != 0.0 || L->cfg.Fy != 0.0);
+    s.Fx_lag = L->Fx_lag;
+    s.Fy_lag = L->Fy_lag;
+    return s;
+}
+
+template <typename T>
+static int launch_step(lbmx_lattice *L)
+{
+    T *S = (T *)L->pop[L->cur];
+    T *D = (T *)L->pop[L->cur ^ 1];
+    const Geo g = lbmx_geo(L);
+    const StepFlags s = lbmx_flags(L);
+    const Fields F{L->R, L->U, L->V};
+
+    BulkArgs ba;
+    ba.g = g;
+    ba.s = s;
+    ba.F = F;
+    ba.tiles_per_row = (L->Nx + kBulkThreads - 1) / kBulkThreads;
+    ba.ntiles = (long long)ba.tiles_per_row * L->Ny;
+    ba.nxcd = L->nxcd;
+    ba.ibb = L->cfg.use_ibb;
+    int grid = L->bulk_grid;
+    if ((long long)grid > ba.ntiles) grid = (int)((ba.ntiles + 7) / 8 * 8);
+    if (grid < 8) grid = 8;
+    switch (L->geom) {
+    case 0: hipLaunchKernelGGL((k_bulk<T, 0>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S, D, ba); break;
+    case 1: hipLaunchKernelGGL((k_bulk<T, 1>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S, D, ba); break;
+    default: hipLaunchKernelGGL((k_bulk<T, 2>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S, D, ba); break;
+    }
+    HIPCHK(hipGetLastError());
+
+    if (L->nlinks) {
+        LinkArgs la;
+        la.g = g;
+        la.s = s;
+        la.F = F;
+        la.links = L->links;
+        la.nlinks = L->nlinks;
+        la.ibb = L->cfg.use_ibb;
+        la.write_bb = 0;
+        la.read_val = 0;
+        la.partials = L->partials;
+        hipLaunchKernelGGL(k_links<T>, dim3(L->npartials), dim3(kLinkThreads), 0, L->stream,
+                           (const T *)S, D, la);
+        HIPCHK(hipGetLastError());
+    }
+    if (L->nprog || L->nlinks) {
+        BcArgs bc;
+        bc.g = g;
+        bc.s = s;
+        bc.F = F;
+        bc.bcs = L->bc_dev;
+        bc.nbc = L->nprog;
+        bc.partials = L->partials;
+        bc.npartials = L->nlinks ? L->npartials : 0;
+        bc.force_out = L->force_dev;
+        hipLaunchKernelGGL(k_bc<T>, dim3(1), dim3(kBcThreads), 0, L->stream, (const T *)S, D,
+                           bc);
+        HIPCHK(hipGetLastError());
+    }
+    if (!s.exact && (L->edge_rows + L->edge_cols) > 0) {
+        EdgeArgs ea;
+        ea.g = g;
+        ea.s = s;
+        ea.F = F;
+        ea.nrow = L->edge_rows;
+        ea.ncol = L->edge_cols;
+        const int ne = ea.nrow + ea.ncol;
+        hipLaunchKernelGGL(k_edges<T>, dim3((ne + 255) / 256), dim3(256), 0, L->stream, S, ea);
+        HIPCHK(hipGetLastError());
+    }
+    L->cur ^= 1;
+    L->lag_arrays = 0;
+    L->Fx_lag = L->cfg.Fx;
+    L->Fy_lag = L->cfg.Fy;
+    L->steps++;
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!L->has_state) return lbmx_fail(LBMX_ERR_STATE, "no state uploaded");
+    for (size_t b = 0; b < L->vals_set.size(); b++)
+        if (lbmx_bc_nvals(L->types[b]) && !L->vals_set[b])
+            return lbmx_fail(LBMX_ERR_STATE, "boundary %zu has no values", b);
+    for (int i = 0; i < nsteps; i++) {
+        int rc = (L->esize == 8) ? launch_step<double>(L) : launch_step<float>(L);
+        if (rc) return rc;
+    }
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_forces(lbmx_lattice *L, double *Lift, double *Drag)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    HIPCHK(hipMemcpyAsync(L->force_host, L->force_dev, 2 * sizeof(double),
+                          hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    if (Drag) *Drag = L->force_host[0];
+    if (Lift) *Lift = L->force_host[1];
+    return LBMX_OK;
+}
+
+int lbmx_launch_moments(lbmx_lattice *L)
+{
+    MomArgs m;
+    m.g = lbmx_geo(L);
+    m.F = Fields{L->R, L->U, L->V};
+    m.Fx = L->cfg.Fx;
+    m.Fy = L->cfg.Fy;
+    dim3 grid((L->Nx + 255) / 256, L->Ny);
+    if (L->esize == 8)
+        hipLaunchKernelGGL(k_moments<double>, grid, dim3(256), 0, L->stream,
+                           (const double *)L->pop[L->cur], m);
+    else
+        hipLaunchKernelGGL(k_moments<float>, grid, dim3(256), 0, L->stream,
+                           (const float *)L->pop[L->cur], m);
+    HIPCHK(hipGetLastError());
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_moments(lbmx_lattice *L, double *rho, double *u, double *v)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!L->has_state) return lbmx_fail(LBMX_ERR_STATE, "no state uploaded");
+    if (!L->lag_arrays) {
+        int rc = lbmx_launch_moments(L);
+        if (rc) return rc;
+    }
+    int rc = LBMX_OK;
+    if (rho) rc = lbmx_get_field(L, rho, L->R);
+    if (!rc && u) rc = lbmx_get_field(L, u, L->U);
+    if (!rc && v) rc = lbmx_get_field(L, v, L->V);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_synchronize(lbmx_lattice *L)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return LBMX_OK;
+}
+
+extern "C" int64_t lbmx_steps_done(const lbmx_lattice *L) { return L ? L->steps : 0; }
+extern "C" int64_t lbmx_device_bytes(const lbmx_lattice *L) { return L ? L->bytes : 0; }

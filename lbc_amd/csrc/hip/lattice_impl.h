@@ -1,0 +1,72 @@
+// lattice_impl.h -- internal definition of lbmx_lattice and shared host helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "lbmx.h"
+#include "step_kernels.h"
+
+constexpr int kQ = 9;
+
+int lbmx_fail(int code, const char *fmt, ...);
+int lbmx_device_info(int *ncu);
+int lbmx_bc_on_column(int t);
+int lbmx_bc_nvals(int t);
+int lbmx_bc_check(int t, int index, int Nx, int Ny);
+
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess)                                                      \
+            return lbmx_fail(LBMX_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+struct lbmx_lattice {
+    lbmx_config cfg{};
+    int Nx = 0, Ny = 0, pitch = 0;
+    size_t plane = 0;      // elements per population plane
+    size_t esize = 8;      // bytes per stored population
+    hipStream_t stream = nullptr;
+    void *pop[2] = {nullptr, nullptr};
+    int cur = 0;           // buffer holding the state the next step starts from
+    double *R = nullptr, *U = nullptr, *V = nullptr;
+
+    // geometry
+    int geom = 0;
+    uint8_t *cls = nullptr;
+    double *lut = nullptr;
+    double *om_arr = nullptr;
+    double om_uniform = 1.0;
+    long long nsolid = 0;
+    lbmx::Link *links = nullptr;
+    int nlinks = 0;
+    double *partials = nullptr;
+    int npartials = 0;
+    double *force_dev = nullptr;
+    double *force_host = nullptr;
+    int edge_rows = 0, edge_cols = 0;
+
+    // boundary table (as given) and the device program compiled from it
+    std::vector<int> types, index, vals_set;
+    std::vector<double *> val1, val2;
+    lbmx::BcDesc *bc_dev = nullptr;
+    int nprog = 0;
+
+    int has_geometry = 0, has_state = 0, lag_arrays = 0;
+    double Fx_lag = 0.0, Fy_lag = 0.0;
+    int64_t steps = 0;
+    int ncu = 0, nxcd = 8, bulk_grid = 0;
+    int64_t bytes = 0;
+    double *staging = nullptr;
+    size_t staging_bytes = 0;
+};
+
+lbmx::Geo lbmx_geo(const lbmx_lattice *L);
+lbmx::StepFlags lbmx_flags(const lbmx_lattice *L);
+int lbmx_put_pops(lbmx_lattice *L, int which, const double *aos);
+int lbmx_get_pops(lbmx_lattice *L, int which, double *aos);
+int lbmx_put_field(lbmx_lattice *L, double *dev, const double *host);
+int lbmx_get_field(lbmx_lattice *L, double *host, const double *dev);
+int lbmx_launch_moments(lbmx_lattice *L);

@@ -1,0 +1,798 @@
+// step_kernels.h -- gfx950 kernels of one lbc time step on a resident lattice.
+//
+// Data layout in HBM (structure of arrays): population k of node (x, y) lives at
+// P[k*plane + y*pitch + x], with pitch a multiple of 64 elements and a small
+// gap between planes.  Two population buffers S (state at the start of the step,
+// the reference's `f`) and D (the reference's `f_new`) swap roles every step.
+//
+// One step (src/core/main_lbm.c:31-61) is
+//   k_bulk      every fluid node: lagged moments, BGK+Guo collision, push of the
+//               post-collision populations to the neighbours (streaming.c pull
+//               rules), half-way bounce-back into its own slots (bounce_back.c)
+//   k_links     solid links: interpolated bounce-back (interpolated_bounce_back.c)
+//               and momentum-exchange force partial sums (compute_force_mem.c)
+//   k_bc        the boundary table in order, one workgroup (main_lbm.c:49-58),
+//               then the fixed-order force total
+//   k_edges     stale-entry bookkeeping at non-periodic domain edges
+// and k_moments produces compute_macroscopic_fields' output when it is asked for.
+//
+// Semantics reproduced exactly:
+//   * collision uses LAGGED moments, i.e. the rho/u/v arrays left by the previous
+//     step (collision.c:24-32).  They equal the moments of S for every step after
+//     the first, so they are recomputed from S in registers; the first step after
+//     an upload reads the uploaded arrays instead.
+//   * streaming skips out-of-domain and solid sources (streaming.c:36-40); those
+//     D entries keep their old content, which in the reference is the
+//     post-collision value of the previous step at the same node.  The push
+//     kernel never writes them; k_edges stores that post-collision value into
+//     the stale slots of S before S becomes the next step's D.
+//   * solid nodes are never written in either buffer.
+//   * boundary conditions read lagged moments and post-collision `f`
+//     (periodic_x.c) which are recomputed from the untouched S on demand.
+// With LBMX_EXACT_STATE the bulk kernel also writes the post-collision values
+// in place into S (and the lagged moments into R/U/V), so both buffers hold the
+// reference's full state after every step.
+#pragma once
+#include <math.h>
+#include "d2q9.h"
+
+namespace lbmx {
+
+struct Geo {
+    int Nx, Ny, pitch;
+    size_t plane;          // elements per population plane (incl. gap)
+    int geom;              // 0: all fluid, uniform omega; 1: class byte + omega LUT;
+                           // 2: class byte + per-node omega array
+    const uint8_t *cls;    // 0 = solid, >= 1 fluid class
+    const double *lut;     // omega per class
+    const double *om_arr;  // per-node omega (geom 2)
+    double om;             // uniform omega (geom 0)
+    int per_x, per_y;
+    double rho0;           // shift density of FP32 storage
+};
+
+struct StepFlags {
+    int lag_arrays;        // lagged moments come from R/U/V (first step after upload)
+    int exact;             // exact-state mode: S holds post-collision values after k_bulk
+    int force;             // body force is non-zero
+    double Fx, Fy;         // body force of this step
+    double Fx_lag, Fy_lag; // body force the lagged moments were computed with
+};
+
+struct Fields {
+    double *R, *U, *V;     // pitched moment arrays
+};
+
+__device__ __forceinline__ bool solid_at(const Geo &g, size_t n)
+{
+    return g.geom != 0 && g.cls[n] == 0;
+}
+
+__device__ __forceinline__ double omega_at(const Geo &g, size_t n)
+{
+    return g.geom == 0 ? g.om : (g.geom == 1 ? g.lut[g.cls[n]] : g.om_arr[n]);
+}
+
+// ---- population storage ----------------------------------------------------
+// FP64 stores f itself; FP32-shifted stores g = f - w*rho0 (Lehmann et al.,
+// Phys. Rev. E 106, 015308 (2022), "DDF shifting").
+__device__ __forceinline__ double pop_get(const double *P, size_t i, int k, double)
+{
+    return P[i];
+}
+__device__ __forceinline__ double pop_get(const float *P, size_t i, int k, double rho0)
+{
+    return (double)P[i] + c_w[k] * rho0;
+}
+__device__ __forceinline__ void pop_put(double *P, size_t i, int k, double v, double)
+{
+    P[i] = v;
+}
+__device__ __forceinline__ void pop_put(float *P, size_t i, int k, double v, double rho0)
+{
+    P[i] = (float)(v - c_w[k] * rho0);
+}
+
+// ---- FP32-shifted collision ------------------------------------------------
+// g_eq = feq - w*rho0 = w*(drho + rho*(3cu + 4.5cu^2 - 1.5u^2)), so the O(1)
+// part of every population cancels exactly and FP32 carries only deviations.
+struct MomentsF {
+    float dr, ux, uy;
+};
+
+__device__ __forceinline__ MomentsF moments_f32(const float g[Q], float rho0, float Fx,
+                                                float Fy)
+{
+#pragma clang fp contract(fast)
+    float dr = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
+    float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
+    float jy = (g[2] - g[4]) + (g[5] + g[6]) - (g[7] + g[8]);
+    float rho = rho0 + dr;
+    if (rho < 1e-16f) rho = 1e-16f;
+    float inv = 1.0f / rho;
+    MomentsF m;
+    m.dr = rho - rho0;
+    m.ux = (jx + 0.5f * Fx) * inv;
+    m.uy = (jy + 0.5f * Fy) * inv;
+    return m;
+}
+
+__device__ __forceinline__ void collide_f32(float g[Q], const MomentsF &m, float rho0, float om,
+                                            bool force, float Fx, float Fy)
+{
+#pragma clang fp contract(fast)
+    float rho = rho0 + m.dr;
+    float usq15 = 1.5f * (m.ux * m.ux + m.uy * m.uy);
+    float uF3 = 3.0f * (m.ux * Fx + m.uy * Fy);
+    float gf = 1.0f - 0.5f * om;
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const float w = (float)D2Q9::w[k];
+        const float cx = (float)D2Q9::cx[k], cy = (float)D2Q9::cy[k];
+        float cu = cx * m.ux + cy * m.uy;
+        float geq = w * (m.dr + rho * (cu * (3.0f + 4.5f * cu) - usq15));
+        float r = g[k] - om * (g[k] - geq);
+        if (force) {
+            float cF = cx * Fx + cy * Fy;
+            r += gf * w * (3.0f * cF + 9.0f * cu * cF - uF3);
+        }
+        g[k] = r;
+    }
+}
+
+// ---- lagged moments and post-collision values on demand --------------------
+__device__ __forceinline__ Moments lagged_at(const double *S, const Geo &g, const Fields &F,
+                                             const StepFlags &s, size_t n)
+{
+    if (s.lag_arrays || s.exact) return Moments{F.R[n], F.U[n], F.V[n]};
+    if (solid_at(g, n)) return Moments{NAN, NAN, NAN};
+    double f[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) f[k] = S[k * g.plane + n];
+    return moments_ref(f, s.Fx_lag, s.Fy_lag);
+}
+
+__device__ __forceinline__ Moments lagged_at(const float *S, const Geo &g, const Fields &F,
+                                             const StepFlags &s, size_t n)
+{
+    if (s.lag_arrays || s.exact) return Moments{F.R[n], F.U[n], F.V[n]};
+    if (solid_at(g, n)) return Moments{NAN, NAN, NAN};
+    float q[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) q[k] = S[k * g.plane + n];
+    MomentsF m = moments_f32(q, (float)g.rho0, (float)s.Fx_lag, (float)s.Fy_lag);
+    return Moments{(double)m.dr + g.rho0, (double)m.ux, (double)m.uy};
+}
+
+// Post-collision populations of node n (reference values, FP64), i.e. what the
+// reference's `f` holds at n after collision.c.  Solid nodes are not collided.
+__device__ __forceinline__ void star_node(const double *S, const Geo &g, const Fields &F,
+                                          const StepFlags &s, size_t n, double out[Q])
+{
+#pragma unroll
+    for (int k = 0; k < Q; k++) out[k] = S[k * g.plane + n];
+    if (s.exact || solid_at(g, n)) return;
+    Moments m = s.lag_arrays ? Moments{F.R[n], F.U[n], F.V[n]}
+                             : moments_ref(out, s.Fx_lag, s.Fy_lag);
+    collide_ref(out, m, omega_at(g, n), s.force, s.Fx, s.Fy);
+}
+
+__device__ __forceinline__ void star_node(const float *S, const Geo &g, const Fields &F,
+                                          const StepFlags &s, size_t n, double out[Q])
+{
+    float q[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) q[k] = S[k * g.plane + n];
+    if (!(s.exact || solid_at(g, n))) {
+        const float r0 = (float)g.rho0;
+        MomentsF m;
+        if (s.lag_arrays) {
+            m.dr = (float)(F.R[n] - g.rho0);
+            m.ux = (float)F.U[n];
+            m.uy = (float)F.V[n];
+        } else {
+            m = moments_f32(q, r0, (float)s.Fx_lag, (float)s.Fy_lag);
+        }
+        collide_f32(q, m, r0, (float)omega_at(g, n), s.force, (float)s.Fx, (float)s.Fy);
+    }
+#pragma unroll
+    for (int k = 0; k < Q; k++) out[k] = (double)q[k] + D2Q9::w[k] * g.rho0;
+}
+
+// ---- k_bulk ------------------------------------------------------------------
+// Persistent grid: blocks are dealt to the 8 XCDs round-robin, so block b is
+// given chunk (b % 8) * (G / 8) + b / 8 and each XCD sweeps one contiguous band
+// of rows; within a block, tiles (TX consecutive nodes of one row) are visited in
+// row-major order, so the shifted stores of neighbouring tiles meet in one L2.
+constexpr int kBulkThreads = 256;
+
+struct BulkArgs {
+    Geo g;
+    StepFlags s;
+    Fields F;
+    int tiles_per_row;
+    long long ntiles;
+    int nxcd;
+    int ibb;               // solid links are written by k_links (interpolated BB)
+};
+
+template <typename T, int GEOM>
+__global__ void __launch_bounds__(kBulkThreads) k_bulk(T *__restrict__ S, T *__restrict__ D,
+                                                      BulkArgs a)
+{
+    const Geo &g = a.g;
+    const int G = gridDim.x;
+    const int per_xcd = G / a.nxcd;
+    const int b = blockIdx.x;
+    const long long chunk = (long long)(b % a.nxcd) * per_xcd + b / a.nxcd;
+    const long long t0 = a.ntiles * chunk / G, t1 = a.ntiles * (chunk + 1) / G;
+    const size_t P = g.plane;
+
+    for (long long t = t0; t < t1; t++) {
+        const int y = (int)(t / a.tiles_per_row);
+        const int x = (int)(t % a.tiles_per_row) * kBulkThreads + threadIdx.x;
+        if (x >= g.Nx) continue;
+        const size_t n = (size_t)y * g.pitch + x;
+
+        double om;
+        if constexpr (GEOM == 0) {
+            om = g.om;
+        } else {
+            const uint8_t c = g.cls[n];
+            if (c == 0) {
+                if (a.s.exact && !a.s.lag_arrays) {
+                    a.F.R[n] = NAN;
+                    a.F.U[n] = NAN;
+                    a.F.V[n] = NAN;
+                }
+                continue;
+            }
+            om = (GEOM == 1) ? g.lut[c] : g.om_arr[n];
+        }
+
+        T p[Q];
+#pragma unroll
+        for (int k = 0; k < Q; k++) p[k] = S[k * P + n];
+
+        if constexpr (sizeof(T) == 8) {
+            double *f = reinterpret_cast<double *>(p);
+            Moments m;
+            if (a.s.lag_arrays) {
+                m = Moments{a.F.R[n], a.F.U[n], a.F.V[n]};
+            } else {
+                m = moments_ref(f, a.s.Fx_lag, a.s.Fy_lag);
+                if (a.s.exact) {
+                    a.F.R[n] = m.rho;
+                    a.F.U[n] = m.ux;
+                    a.F.V[n] = m.uy;
+                }
+            }
+            collide_ref(f, m, om, a.s.force, a.s.Fx, a.s.Fy);
+        } else {
+            float *q = reinterpret_cast<float *>(p);
+            const float r0 = (float)g.rho0;
+            MomentsF m;
+            if (a.s.lag_arrays) {
+                m.dr = (float)(a.F.R[n] - g.rho0);
+                m.ux = (float)a.F.U[n];
+                m.uy = (float)a.F.V[n];
+            } else {
+                m = moments_f32(q, r0, (float)a.s.Fx_lag, (float)a.s.Fy_lag);
+                if (a.s.exact) {
+                    a.F.R[n] = (double)m.dr + g.rho0;
+                    a.F.U[n] = m.ux;
+                    a.F.V[n] = m.uy;
+                }
+            }
+            collide_f32(q, m, r0, (float)om, a.s.force, (float)a.s.Fx, (float)a.s.Fy);
+        }
+
+        if (a.s.exact) {
+#pragma unroll
+            for (int k = 0; k < Q; k++) S[k * P + n] = p[k];
+        }
+
+        // push: D[k][x + c_k] = f*_k, wrapping or dropping at the domain edge
+#pragma unroll
+        for (int k = 0; k < Q; k++) {
+            int xd = x + D2Q9::cx[k], yd = y + D2Q9::cy[k];
+            long long off = (long long)D2Q9::cy[k] * g.pitch + D2Q9::cx[k];
+            if (D2Q9::cx[k] != 0) {
+                if (xd < 0) {
+                    if (!g.per_x) continue;
+                    off += g.Nx;
+                } else if (xd >= g.Nx) {
+                    if (!g.per_x) continue;
+                    off -= g.Nx;
+                }
+            }
+            if (D2Q9::cy[k] != 0) {
+                if (yd < 0) {
+                    if (!g.per_y) continue;
+                    off += (long long)g.Ny * g.pitch;
+                } else if (yd >= g.Ny) {
+                    if (!g.per_y) continue;
+                    off -= (long long)g.Ny * g.pitch;
+                }
+            }
+            const size_t nd = (size_t)((long long)n + off);
+            if constexpr (GEOM != 0) {
+                if (k != 0 && g.cls[nd] == 0) {
+                    // solid destination: half-way bounce-back into the own slot
+                    // opp(k) (bounce_back.c:43-46); with interpolated bounce-back
+                    // k_links writes that slot instead.
+                    if (!a.ibb) D[D2Q9::opp[k] * P + n] = p[k];
+                    continue;
+                }
+            }
+            D[k * P + nd] = p[k];
+        }
+    }
+}
+
+
+// Select element k (runtime) of a small register array without scratch.
+__device__ __forceinline__ double sel9(const double a[Q], int k)
+{
+    double r = a[0];
+#pragma unroll
+    for (int i = 1; i < Q; i++) r = (k == i) ? a[i] : r;
+    return r;
+}
+
+// ---- k_links -----------------------------------------------------------------
+// One thread per fluid->solid link (x, k) in row-major order of x, k ascending.
+struct Link {
+    long long n;    // fluid node
+    long long nb;   // behind node x - c_k, or -1 when it lies outside a
+                    // non-periodic domain (plain bounce-back then)
+    double q;       // wall distance fraction, clamped (interpolated_bounce_back.c:51-61)
+    int k;          // direction pointing into the solid
+    int bsolid;     // behind node is solid
+};
+
+constexpr int kLinkThreads = 256;
+
+struct LinkArgs {
+    Geo g;
+    StepFlags s;
+    Fields F;
+    const Link *links;
+    int nlinks;
+    int ibb;
+    int write_bb;       // also store plain bounce-back values (stand-alone bounce_back)
+    int read_val;       // force only: take f_new(x, opp k) from D instead of computing it
+    double *partials;   // 2 doubles per block
+};
+
+__device__ __forceinline__ void block_sum2(double &fx, double &fy, double *lds)
+{
+    // fixed butterfly within each wavefront, then waves summed in order
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        fx += __shfl_xor(fx, o, 64);
+        fy += __shfl_xor(fy, o, 64);
+    }
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+        lds[2 * wave] = fx;
+        lds[2 * wave + 1] = fy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sx = 0.0, sy = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / 64); w++) {
+            sx += lds[2 * w];
+            sy += lds[2 * w + 1];
+        }
+        fx = sx;
+        fy = sy;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kLinkThreads) k_links(const T *__restrict__ S,
+                                                        T *__restrict__ D, LinkArgs a)
+{
+    __shared__ double lds[2 * kLinkThreads / 64];
+    const int l = blockIdx.x * kLinkThreads + threadIdx.x;
+    double fx = 0.0, fy = 0.0;
+    if (l < a.nlinks) {
+        const Link L = a.links[l];
+        const int k = L.k, ko = c_opp[k];
+        double st[Q];
+        star_node(S, a.g, a.F, a.s, (size_t)L.n, st);
+        const double fk = sel9(st, k);
+        double val = fk;
+        if (a.read_val) {
+            val = pop_get(D, (size_t)ko * a.g.plane + (size_t)L.n, ko, a.g.rho0);
+        } else if (a.ibb && L.nb >= 0) {
+            const double q = L.q;
+            double sb[Q];
+            star_node(S, a.g, a.F, a.s, (size_t)L.nb, sb);
+            if (q <= 0.5)
+                val = (1.0 - 2.0 * q) * fk + 2.0 * q * sel9(sb, k);
+            else if (!L.bsolid)
+                val = (1.0 / (2.0 * q)) * fk + (2.0 * q - 1.0) / (2.0 * q) * sel9(sb, ko);
+        }
+        if ((a.ibb || a.write_bb) && !a.read_val)
+            pop_put(D, (size_t)ko * a.g.plane + (size_t)L.n, ko, val, a.g.rho0);
+        // compute_force_mem.c:55-57: f*(x,k) + f_new(x,opp k) after (I)BB
+        const double df = fk + val;
+        fx = df * (double)c_cx[k];
+        fy = df * (double)c_cy[k];
+    }
+    block_sum2(fx, fy, lds);
+    if (threadIdx.x == 0) {
+        a.partials[2 * blockIdx.x] = fx;
+        a.partials[2 * blockIdx.x + 1] = fy;
+    }
+}
+
+// ---- k_bc: the boundary table, in order, in one workgroup ----------------------
+struct BcDesc {
+    int type, index;
+    const double *v1, *v2;
+};
+
+constexpr int kBcThreads = 1024;
+
+struct BcArgs {
+    Geo g;
+    StepFlags s;
+    Fields F;
+    const BcDesc *bcs;
+    int nbc;
+    const double *partials;
+    int npartials;
+    double *force_out;   // [0] = x (the reference's D), [1] = y (L)
+};
+
+template <typename T>
+__device__ void bc_zou_he_column(const T *S, T *D, const BcArgs &a, const BcDesc &d)
+{
+    // velocity_inlet.c:28-39, velocity_outlet.c:28-39, pressure_inlet.c:25-39,
+    // pressure_outlet.c:25-39
+    const Geo &g = a.g;
+    const size_t P = g.plane;
+    const int i = d.index;
+    const bool inlet = (d.type == 7 || d.type == 3);
+    const bool pressure = (d.type == 3 || d.type == 4);
+    for (int j = threadIdx.x; j < g.Ny; j += blockDim.x) {
+        const size_t n = (size_t)j * g.pitch + i;
+        if (solid_at(g, n)) continue;
+        double f[Q];
+#pragma unroll
+        for (int k = 0; k < Q; k++) f[k] = pop_get(D, k * P + n, k, g.rho0);
+        double r, U, V;
+        if (inlet) {
+            if (!pressure) {
+                U = d.v1[j];
+                V = d.v2[j];
+                r = (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[6] + f[7])) / (1.0 - U);
+            } else {
+                r = d.v1[j];
+                U = 1.0 - (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[6] + f[7])) / r;
+                V = lagged_at(S, g, a.F, a.s, n + 1).uy;
+            }
+            pop_put(D, 1 * P + n, 1, f[3] + (2.0 / 3.0) * r * U, g.rho0);
+            pop_put(D, 5 * P + n, 5,
+                    f[7] + (1.0 / 6.0) * r * U - 0.5 * (f[2] - f[4]) + (1.0 / 2.0) * r * V,
+                    g.rho0);
+            pop_put(D, 8 * P + n, 8,
+                    f[6] + (1.0 / 6.0) * r * U + 0.5 * (f[2] - f[4]) - (1.0 / 2.0) * r * V,
+                    g.rho0);
+        } else {
+            if (!pressure) {
+                U = d.v1[j];
+                V = d.v2[j];
+                r = (f[0] + f[2] + f[4] + 2.0 * (f[1] + f[5] + f[8])) / (1.0 + U);
+            } else {
+                r = d.v1[j];
+                U = (f[0] + f[2] + f[4] + 2.0 * (f[1] + f[5] + f[8])) / r - 1.0;
+                V = lagged_at(S, g, a.F, a.s, n - 1).uy;
+            }
+            pop_put(D, 3 * P + n, 3, f[1] - (2.0 / 3.0) * r * U, g.rho0);
+            pop_put(D, 6 * P + n, 6,
+                    f[8] - 0.5 * (f[2] - f[4]) - (1.0 / 6.0) * r * U + (1.0 / 2.0) * r * V,
+                    g.rho0);
+            pop_put(D, 7 * P + n, 7,
+                    f[5] + 0.5 * (f[2] - f[4]) - (1.0 / 6.0) * r * U - (1.0 / 2.0) * r * V,
+                    g.rho0);
+        }
+    }
+}
+
+template <typename T>
+__device__ void bc_zou_he_row(T *D, const BcArgs &a, const BcDesc &d)
+{
+    // velocity_top.c:28-39 and velocity_bottom.c:28-39 (the latter stores its
+    // third closure into slot 7, leaving slot 6 as streamed)
+    const Geo &g = a.g;
+    const size_t P = g.plane;
+    const int j = d.index;
+    const bool top = (d.type == 9);
+    for (int i = threadIdx.x; i < g.Nx; i += blockDim.x) {
+        const size_t n = (size_t)j * g.pitch + i;
+        if (solid_at(g, n)) continue;
+        const double U = d.v1[i], V = d.v2[i];
+        double f[Q];
+#pragma unroll
+        for (int k = 0; k < Q; k++) f[k] = pop_get(D, k * P + n, k, g.rho0);
+        if (top) {
+            double r = (f[0] + f[1] + f[3] + 2 * (f[2] + f[5] + f[6])) / (1 + V);
+            pop_put(D, 4 * P + n, 4, f[2] - 2.0 / 3.0 * r * V, g.rho0);
+            pop_put(D, 7 * P + n, 7,
+                    f[5] + (f[1] - f[3]) / 2.0 - 0.5 * r * U - 1.0 / 6.0 * r * V, g.rho0);
+            pop_put(D, 8 * P + n, 8,
+                    f[6] - (f[1] - f[3]) / 2.0 + 0.5 * r * U - 1.0 / 6.0 * r * V, g.rho0);
+        } else {
+            double r = (f[0] + f[1] + f[3] + 2 * (f[4] + f[7] + f[8])) / (1 - V);
+            pop_put(D, 2 * P + n, 2, f[4] + 2.0 / 3.0 * r * V, g.rho0);
+            pop_put(D, 5 * P + n, 5,
+                    f[7] - (f[1] - f[3]) / 2.0 + 0.5 * r * U + 1.0 / 6.0 * r * V, g.rho0);
+            pop_put(D, 7 * P + n, 7,
+                    f[8] + (f[1] - f[3]) / 2.0 - 0.5 * r * U + 1.0 / 6.0 * r * V, g.rho0);
+        }
+    }
+}
+
+template <typename T>
+__device__ void bc_regularized(const T *S, T *D, const BcArgs &a, const BcDesc &d)
+{
+    // src/bc/*_regularized.c:18-71: feq(rho_b, u_b) + 1.5 w Q:Pi with Pi from
+    // f_new - feq(lagged) at the neighbouring interior node
+    const Geo &g = a.g;
+    const size_t P = g.plane;
+    const int t = d.type;
+    const bool column = (t == 11 || t == 12 || t == 15 || t == 16);
+    const bool pressure = (t >= 15 && t <= 18);
+    const int step_in = (t == 12 || t == 14 || t == 16 || t == 18) ? 1 : -1;
+    const int nedge = column ? g.Ny : g.Nx;
+    for (int e = threadIdx.x; e < nedge; e += blockDim.x) {
+        const int ib = column ? d.index : e, jb = column ? e : d.index;
+        const int ii = column ? d.index + step_in : e, ji = column ? e : d.index + step_in;
+        const size_t nb = (size_t)jb * g.pitch + ib, ni = (size_t)ji * g.pitch + ii;
+        if (solid_at(g, nb)) continue;
+        const Moments mi = lagged_at(S, g, a.F, a.s, ni);
+        const double ri = mi.rho, ui = mi.ux, vi = mi.uy;
+        const double usq_i = ui * ui + vi * vi;
+        double rb, Ub, Vb;
+        if (pressure) {
+            rb = d.v1[e];
+            Ub = ui;
+            Vb = vi;
+        } else {
+            rb = ri;
+            Ub = d.v1[e];
+            Vb = d.v2[e];
+        }
+        const double usq_b = Ub * Ub + Vb * Vb;
+        double Pxx = 0.0, Pxy = 0.0, Pyy = 0.0;
+#pragma unroll
+        for (int k = 0; k < Q; k++) {
+            const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
+            double neq = pop_get(D, k * P + ni, k, g.rho0) -
+                         feq_ref(D2Q9::w[k], (double)cx, (double)cy, ri, ui, vi, usq_i);
+            Pxx += (double)(cx * cx) * neq;
+            Pxy += (double)(cx * cy) * neq;
+            Pyy += (double)(cy * cy) * neq;
+        }
+#pragma unroll
+        for (int k = 0; k < Q; k++) {
+            const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
+            double eq = feq_ref(D2Q9::w[k], (double)cx, (double)cy, rb, Ub, Vb, usq_b);
+            double Qxx = (double)(cx * cx) - 1.0 / 3.0;
+            double Qxy = (double)(cx * cy);
+            double Qyy = (double)(cy * cy) - 1.0 / 3.0;
+            double neq = (3.0 / 2.0) * D2Q9::w[k] * (Qxx * Pxx + 2.0 * Qxy * Pxy + Qyy * Pyy);
+            pop_put(D, k * P + nb, k, eq + neq, g.rho0);
+        }
+    }
+}
+
+template <typename T>
+__device__ void bc_convective(T *D, const BcArgs &a, const BcDesc &d)
+{
+    // convective_{inlet,outlet,top,bottom}.c:18-28: copy all nine f_new
+    const Geo &g = a.g;
+    const size_t P = g.plane;
+    const int t = d.type;
+    const bool column = (t == 1 || t == 2);
+    const int from = d.index + ((t == 1 || t == 20) ? 1 : -1);
+    const int nedge = column ? g.Ny : g.Nx;
+    for (int e = threadIdx.x; e < nedge; e += blockDim.x) {
+        const size_t nb = column ? (size_t)e * g.pitch + d.index : (size_t)d.index * g.pitch + e;
+        const size_t ns = column ? (size_t)e * g.pitch + from : (size_t)from * g.pitch + e;
+        if (solid_at(g, nb)) continue;
+#pragma unroll
+        for (int k = 0; k < Q; k++) D[k * P + nb] = D[k * P + ns];
+    }
+}
+
+template <typename T>
+__device__ void bc_periodic(const T *S, T *D, const BcArgs &a, bool along_x)
+{
+    // periodic_x.c:19-50 / periodic_y.c:19-50: re-pull wrapped populations of the
+    // first/last column (row) from post-collision f; the other axis is not wrapped
+    const Geo &g = a.g;
+    const size_t P = g.plane;
+    const int nedge = along_x ? g.Ny : g.Nx;
+    const int L = along_x ? g.Nx : g.Ny;
+    for (int e = threadIdx.x; e < nedge; e += blockDim.x) {
+#pragma unroll
+        for (int k = 1; k < Q; k++) {
+            const int c = along_x ? D2Q9::cx[k] : D2Q9::cy[k];
+            if (c == 0) continue;
+            const int dst = c > 0 ? 0 : L - 1;
+            const int src = c > 0 ? L - c : -c - 1;
+            int i, j, is, js;
+            if (along_x) {
+                i = dst; j = e; is = src; js = e - D2Q9::cy[k];
+                if (js < 0 || js >= g.Ny) continue;
+            } else {
+                i = e; j = dst; is = e - D2Q9::cx[k]; js = src;
+                if (is < 0 || is >= g.Nx) continue;
+            }
+            const size_t nd = (size_t)j * g.pitch + i, ns = (size_t)js * g.pitch + is;
+            if (solid_at(g, nd) || solid_at(g, ns)) continue;
+            double st[Q];
+            star_node(S, g, a.F, a.s, ns, st);
+            pop_put(D, k * P + nd, k, st[k], g.rho0);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBcThreads) k_bc(const T *__restrict__ S, T *__restrict__ D,
+                                                   BcArgs a)
+{
+    for (int b = 0; b < a.nbc; b++) {
+        const BcDesc d = a.bcs[b];
+        switch (d.type) {
+        case 3: case 4: case 7: case 8: bc_zou_he_column(S, D, a, d); break;
+        case 9: case 10: bc_zou_he_row(D, a, d); break;
+        case 1: case 2: case 19: case 20: bc_convective(D, a, d); break;
+        case 5: bc_periodic(S, D, a, true); break;
+        case 6: bc_periodic(S, D, a, false); break;
+        default: bc_regularized(S, D, a, d); break;
+        }
+        __syncthreads();
+    }
+    if (a.npartials > 0 && threadIdx.x == 0) {
+        double fx = 0.0, fy = 0.0;
+        for (int p = 0; p < a.npartials; p++) {
+            fx += a.partials[2 * p];
+            fy += a.partials[2 * p + 1];
+        }
+        a.force_out[0] = fx;
+        a.force_out[1] = fy;
+    }
+}
+
+// ---- k_edges: stale slots of non-periodic domain edges ----------------------------
+// The reference leaves D entries whose streaming source is outside the domain
+// untouched (streaming.c:36-37); they hold the previous step's post-collision
+// value at the same node.  S becomes the next step's D, so its stale slots get
+// that value here, after every reader of S in this step has run.
+struct EdgeArgs {
+    Geo g;
+    StepFlags s;
+    Fields F;
+    int nrow, ncol;   // edge nodes enumerated from rows, then from columns
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_edges(T *__restrict__ S, EdgeArgs a)
+{
+    const Geo &g = a.g;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nrow + a.ncol) return;
+    int x, y;
+    if (e < a.nrow) {   // rows 0 and Ny-1 when y is not periodic
+        y = (e < g.Nx) ? 0 : g.Ny - 1;
+        x = e % g.Nx;
+    } else {            // columns 0 and Nx-1 when x is not periodic (corners excluded
+                        // when the rows are already listed)
+        const int c = e - a.nrow;
+        const int j0 = g.per_y ? 0 : 1;
+        const int len = g.per_y ? g.Ny : g.Ny - 2;
+        x = (c < len) ? 0 : g.Nx - 1;
+        y = j0 + c % len;
+    }
+    const size_t n = (size_t)y * g.pitch + x;
+    if (solid_at(g, n)) return;
+    double st[Q];
+    star_node(S, g, a.F, a.s, n, st);
+#pragma unroll
+    for (int k = 1; k < Q; k++) {
+        const int xs = x - D2Q9::cx[k], ys = y - D2Q9::cy[k];
+        const bool out = (!g.per_x && (xs < 0 || xs >= g.Nx)) ||
+                         (!g.per_y && (ys < 0 || ys >= g.Ny));
+        if (out) pop_put(S, k * g.plane + n, k, st[k], g.rho0);
+    }
+}
+
+// ---- k_moments: compute_macroscopic_fields.c:19-48 --------------------------------
+struct MomArgs {
+    Geo g;
+    Fields F;
+    double Fx, Fy;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_moments(const T *__restrict__ Dn, MomArgs a)
+{
+    const Geo &g = a.g;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= g.Nx) return;
+    const size_t n = (size_t)y * g.pitch + x;
+    double f[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) f[k] = pop_get(Dn, k * g.plane + n, k, g.rho0);
+    if (solid_at(g, n)) {
+        a.F.R[n] = NAN;
+        a.F.U[n] = NAN;
+        a.F.V[n] = NAN;
+        return;
+    }
+    const Moments m = moments_ref(f, a.Fx, a.Fy);
+    a.F.R[n] = m.rho;
+    a.F.U[n] = m.ux;
+    a.F.V[n] = m.uy;
+}
+
+// ---- device-side Taylor-Green initial state ---------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_init_tgv(T *__restrict__ A, T *__restrict__ B,
+                                                  Fields F, Geo g, double u0, double kx,
+                                                  double ky)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= g.Nx) return;
+    const size_t n = (size_t)y * g.pitch + x;
+    const double r = kx / ky;
+    const double ux = u0 * sin(kx * x) * cos(ky * y);
+    const double uy = -u0 * r * cos(kx * x) * sin(ky * y);
+    const double rho = 1.0 - 0.75 * u0 * u0 * (cos(2.0 * kx * x) + r * r * cos(2.0 * ky * y));
+    const double usq = ux * ux + uy * uy;
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const double fe = feq_ref(D2Q9::w[k], (double)D2Q9::cx[k], (double)D2Q9::cy[k], rho,
+                                  ux, uy, usq);
+        pop_put(A, k * g.plane + n, k, fe, g.rho0);
+        pop_put(B, k * g.plane + n, k, fe, g.rho0);
+    }
+    F.R[n] = rho;
+    F.U[n] = ux;
+    F.V[n] = uy;
+}
+
+// ---- layout conversion between the reference's AoS and the planes ----------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_aos_to_planes(T *__restrict__ Pp,
+                                                       const double *__restrict__ aos, Geo g,
+                                                       int y0, int rows)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)rows * g.Nx) return;
+    const int y = y0 + (int)(t / g.Nx), x = (int)(t % g.Nx);
+    const size_t n = (size_t)y * g.pitch + x;
+#pragma unroll
+    for (int k = 0; k < Q; k++) pop_put(Pp, k * g.plane + n, k, aos[t * Q + k], g.rho0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_planes_to_aos(double *__restrict__ aos,
+                                                       const T *__restrict__ Pp, Geo g,
+                                                       int y0, int rows)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)rows * g.Nx) return;
+    const int y = y0 + (int)(t / g.Nx), x = (int)(t % g.Nx);
+    const size_t n = (size_t)y * g.pitch + x;
+#pragma unroll
+    for (int k = 0; k < Q; k++) aos[t * Q + k] = pop_get(Pp, k * g.plane + n, k, g.rho0);
+}
+
+}  // namespace lbmx

@@ -1,0 +1,177 @@
+"""ctypes view of the native device-resident C ABI (include/lbmx.h).
+
+Arrays follow the reference's conventions: populations ``[Ny, Nx, 9]`` float64
+(AoS, k fastest), fields ``[Ny, Nx]`` float64, solid mask ``[Ny, Nx]`` int32.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib
+
+F64 = 0
+F32_SHIFTED = 1
+EXACT_STATE = 1
+
+BC_CODES = {
+    "convective_inlet": 1, "convective_outlet": 2, "pressure_inlet": 3,
+    "pressure_outlet": 4, "periodic_x": 5, "periodic_y": 6, "velocity_inlet": 7,
+    "velocity_outlet": 8, "velocity_top": 9, "velocity_bottom": 10,
+    "velocity_outlet_regularized": 11, "velocity_inlet_regularized": 12,
+    "velocity_top_regularized": 13, "velocity_bottom_regularized": 14,
+    "pressure_outlet_regularized": 15, "pressure_inlet_regularized": 16,
+    "pressure_top_regularized": 17, "pressure_bottom_regularized": 18,
+    "convective_top": 19, "convective_bottom": 20,
+}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Config(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("precision", C.c_int),
+                ("per_x", C.c_int), ("per_y", C.c_int), ("use_ibb", C.c_int),
+                ("Fx", C.c_double), ("Fy", C.c_double), ("rho0", C.c_double),
+                ("flags", C.c_uint)]
+
+
+def _d(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+_bound = False
+
+
+def _bind():
+    global _bound
+    if _bound:
+        return lib()
+    L = lib()
+    i, d, vp = C.c_int, C.c_double, C.c_void_p
+    L.lbmx_last_error.restype = C.c_char_p
+    L.lbmx_device_name.restype = C.c_char_p
+    L.lbmx_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+    L.lbmx_destroy.argtypes = [vp]
+    L.lbmx_destroy.restype = None
+    L.lbmx_set_geometry.argtypes = [vp, _ip, _dp, _dp]
+    L.lbmx_set_boundaries.argtypes = [vp, _ip, _ip, i]
+    L.lbmx_set_bc_values.argtypes = [vp, i, _dp, _dp]
+    L.lbmx_upload.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
+    L.lbmx_step.argtypes = [vp, i]
+    L.lbmx_init_taylor_green.argtypes = [vp, d, d, d]
+    L.lbmx_forces.argtypes = [vp, _dp, _dp]
+    L.lbmx_moments.argtypes = [vp, _dp, _dp, _dp]
+    L.lbmx_download.argtypes = [vp, _dp, _dp]
+    L.lbmx_synchronize.argtypes = [vp]
+    L.lbmx_steps_done.argtypes = [vp]
+    L.lbmx_steps_done.restype = C.c_int64
+    L.lbmx_device_bytes.argtypes = [vp]
+    L.lbmx_device_bytes.restype = C.c_int64
+    _bound = True
+    return L
+
+
+class LbmxError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise LbmxError(f"lbmx error {rc}: {lib().lbmx_last_error().decode()}")
+    return rc
+
+
+def device_name():
+    return _bind().lbmx_device_name().decode()
+
+
+class Lattice:
+    """One lattice resident on the GPU (lbmx_lattice)."""
+
+    def __init__(self, Nx, Ny, precision=F64, per_x=0, per_y=0, use_ibb=0, Fx=0.0, Fy=0.0,
+                 rho0=1.0, exact=False):
+        self._L = _bind()
+        self.Nx, self.Ny = int(Nx), int(Ny)
+        cfg = Config(self.Nx, self.Ny, precision, int(per_x), int(per_y), int(use_ibb),
+                     float(Fx), float(Fy), float(rho0), EXACT_STATE if exact else 0)
+        h = C.c_void_p()
+        check(self._L.lbmx_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.bc_names = []
+
+    def close(self):
+        if self.h:
+            self._L.lbmx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_geometry(self, mask, omega, phi=None):
+        self._keep = (np.ascontiguousarray(mask, dtype=np.int32),
+                      np.ascontiguousarray(omega, dtype=np.float64),
+                      None if phi is None else np.ascontiguousarray(phi, dtype=np.float64))
+        check(self._L.lbmx_set_geometry(self.h, _i(self._keep[0]), _d(self._keep[1]),
+                                        _d(self._keep[2])))
+
+    def set_boundaries(self, bcs):
+        """bcs: list of (name, index)."""
+        self.bc_names = [b[0] for b in bcs]
+        t = np.array([BC_CODES[b[0]] for b in bcs] or [0], dtype=np.int32)
+        ix = np.array([int(b[1]) for b in bcs] or [0], dtype=np.int32)
+        check(self._L.lbmx_set_boundaries(self.h, _i(t), _i(ix), len(bcs)))
+
+    def set_bc_values(self, b, val1=None, val2=None):
+        v1 = None if val1 is None else np.ascontiguousarray(val1, dtype=np.float64)
+        v2 = None if val2 is None else np.ascontiguousarray(val2, dtype=np.float64)
+        check(self._L.lbmx_set_bc_values(self.h, int(b), _d(v1), _d(v2)))
+
+    def upload(self, f, f_new, rho, u, v):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (f, f_new, rho, u, v)]
+        check(self._L.lbmx_upload(self.h, *[_d(a) for a in arrs]))
+
+    def init_taylor_green(self, u0, kx, ky):
+        check(self._L.lbmx_init_taylor_green(self.h, float(u0), float(kx), float(ky)))
+
+    def step(self, n=1):
+        check(self._L.lbmx_step(self.h, int(n)))
+
+    def forces(self):
+        Lf, Df = C.c_double(), C.c_double()
+        check(self._L.lbmx_forces(self.h, C.byref(Lf), C.byref(Df)))
+        return Lf.value, Df.value
+
+    def moments(self):
+        rho = np.empty((self.Ny, self.Nx))
+        u = np.empty((self.Ny, self.Nx))
+        v = np.empty((self.Ny, self.Nx))
+        check(self._L.lbmx_moments(self.h, _d(rho), _d(u), _d(v)))
+        return rho, u, v
+
+    def download(self, both=False):
+        f = np.empty((self.Ny, self.Nx, 9))
+        fn = np.empty((self.Ny, self.Nx, 9)) if both else None
+        check(self._L.lbmx_download(self.h, _d(f), _d(fn)))
+        return (f, fn) if both else f
+
+    def synchronize(self):
+        check(self._L.lbmx_synchronize(self.h))
+
+    @property
+    def device_bytes(self):
+        return int(self._L.lbmx_device_bytes(self.h))

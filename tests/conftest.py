@@ -1,0 +1,27 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+    config.addinivalue_line("markers", "slow: long-running check")
+
+
+def pytest_report_header(config):
+    return f"lbc_amd repo: {ROOT}"
+
+
+@pytest.fixture(scope="session")
+def ref_lib():
+    import oracle
+    r = oracle.reference()
+    if r is None:
+        pytest.skip("oracle/_ref/liblbm_ref.so not built (reference tree absent)")
+    return r
