@@ -1,0 +1,224 @@
+"""Benchmark of the lbc D2Q9 hot path on one MI355X.
+
+Headline (``value``): FP64 MLUPS of the fused step on the periodic Taylor-Green
+lattice 16384 x 16384 (tau = 0.8, u0 = 0.01), the top of the roofline sweep of
+the channel/cylinder configuration (1024^2 .. 16384^2).  The lattice is
+initialised on the device; W untimed steps, then K timed steps between two
+stream synchronisations.
+
+Also reported in the same JSON line:
+  roofline      algorithmic bytes (144 B per cell-update in FP64: 9 populations
+                read + 9 written) / the fused kernel's average duration, measured
+                with HIP events on the lattice's stream, vs the 8 TB/s HBM peak;
+                ``traffic`` is the measured HBM bytes per launch from the
+                rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under profiles/
+  cpu_baseline  the reference library (oracle/_ref, compiled from lbc's own
+                sources with its flags) timed on this host through main_lbm on a
+                bounded sample of the same workload
+  secondary     FP32-shifted MLUPS at 16384^2 and the FP64 cylinder channel
+                2048 x 512 (interpolated bounce-back, regularized inlet/outlet)
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--size N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def tgv_lattice(N, precision):
+    from lbc_amd import native
+    lat = native.Lattice(N, N, precision=precision, per_x=1, per_y=1)
+    lat.set_uniform(1.0 / 0.8)
+    lat.set_boundaries([("periodic_x", 0), ("periodic_y", 0)])
+    k = 2.0 * math.pi / N
+    lat.init_taylor_green(0.01, k, k)
+    return lat
+
+
+def time_lattice(lat, steps, warmup):
+    """(wall seconds, device ms from HIP events) for `steps` steps."""
+    lat.step(warmup)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    ms = lat.time_steps(steps)
+    lat.synchronize()
+    return time.perf_counter() - t0, ms
+
+
+def cylinder_lattice(precision):
+    """Channel past a cylinder, 2048 x 512, Re = 100 (configs' channel case)."""
+    from lbc_amd import native
+    Nx, Ny, R, xc, yc, umax = 2048, 512, 32.0, 512.0, 256.0, 0.05
+    nu = umax * 64 / 100.0
+    tau = 3.0 * nu + 0.5
+    j, i = np.mgrid[0:Ny, 0:Nx]
+    mask = ((j == 0) | (j == Ny - 1)).astype(np.int32)
+    d = np.sqrt((i - xc) ** 2 + (j - yc) ** 2) - R
+    mask[d <= 0] = 1
+    phi = np.where((j == 0) | (j == Ny - 1), -1e30, d)
+    lat = native.Lattice(Nx, Ny, precision=precision, per_x=0, per_y=0, use_ibb=1)
+    lat.set_geometry(mask, np.full((Ny, Nx), 1.0 / tau), phi)
+    lat.set_boundaries([("velocity_inlet_regularized", 0),
+                        ("pressure_outlet_regularized", Nx - 1)])
+    y = np.arange(Ny) - 0.5
+    H = Ny - 2.0
+    uin = np.clip(4 * umax * y * (H - y) / H ** 2, 0, None)
+    lat.set_bc_values(0, uin, np.zeros(Ny))
+    lat.set_bc_values(1, np.ones(Ny), None)
+    rho = np.ones((Ny, Nx))
+    zero = np.zeros((Ny, Nx))
+    f = np.broadcast_to(np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4), (Ny, Nx, 9)).copy()
+    lat.upload(f, f, rho, zero, zero)
+    return lat
+
+
+def load_traffic(tag):
+    """HBM bytes per launch of the fused kernel from the committed pmc summary."""
+    p = os.path.join(ROOT, "profiles", f"pmc_{tag}.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as fh:
+            d = json.load(fh)
+        return float(d["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(N, min_seconds=10.0, max_steps=200):
+    """The reference library's main_lbm on a bounded sample of the TGV workload."""
+    import oracle as O
+    ref = O.reference()
+    kind = "reference"
+    lat = O.Lattice(N, N)
+    k = 2.0 * math.pi / N
+    j, i = np.mgrid[0:N, 0:N].astype(np.float64)
+    u0 = 0.01
+    lat.u[:] = u0 * np.sin(k * i) * np.cos(k * j)
+    lat.v[:] = -u0 * np.cos(k * i) * np.sin(k * j)
+    lat.rho[:] = 1.0 - 0.75 * u0 * u0 * (np.cos(2 * k * i) + np.cos(2 * k * j))
+    from lbc_amd.cases import equilibrium
+    lat.f[:] = equilibrium(lat.rho, lat.u, lat.v)
+    lat.f_new[:] = lat.f
+    lat.omega[:] = 1.0 / 0.8
+    bcs = [("periodic_x", 0, None, None), ("periodic_y", 0, None, None)]
+    if ref is None:
+        kind = "port"
+        step = lambda: O.oracle_step(lat, bcs, 1, 1)   # noqa: E731
+    else:
+        step = lambda: ref.step(lat, bcs, 1, 1)        # noqa: E731
+    step()
+    n, t0 = 0, time.perf_counter()
+    while n < max_steps:
+        step()
+        n += 1
+        if time.perf_counter() - t0 >= min_seconds and n >= 3:
+            break
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(N * N * n / dt / 1e6, 2), "unit": "MLUPS", "cores": threads,
+            "kind": kind,
+            "sample": f"main_lbm FP64 periodic Taylor-Green {N}x{N}, {n} steps "
+                      f"({dt:.1f} s), OMP threads={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=16384)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--cpu-size", type=int, default=2048)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "decomposition is described in DESIGN.md and not built", file=sys.stderr)
+        sys.exit(2)
+
+    from lbc_amd import native
+    N = args.size
+    cells = N * N
+    log(f"device: {native.device_name()}")
+
+    lat = tgv_lattice(N, native.F64)
+    wall, dev_ms = time_lattice(lat, args.steps, args.warmup)
+    lat.close()
+    mlups = cells * args.steps / wall / 1e6
+    kernel_ms = dev_ms / args.steps          # one fused kernel per step here
+    achieved = 144.0 * cells / (kernel_ms * 1e-3) / 1e9
+    log(f"FP64 {N}^2: {wall / args.steps * 1e3:.3f} ms/step wall, {kernel_ms:.3f} ms/step "
+        f"events, {mlups:.0f} MLUPS, {achieved:.0f} GB/s")
+
+    secondary = {}
+    if not args.no_secondary:
+        lat = tgv_lattice(N, native.F32_SHIFTED)
+        w32, d32 = time_lattice(lat, args.steps, args.warmup)
+        lat.close()
+        m32 = cells * args.steps / w32 / 1e6
+        secondary["fp32_shifted_tgv"] = {
+            "workload": f"taylor_green_periodic_{N}x{N}_fp32_shifted", "MLUPS": round(m32, 1),
+            "GB_per_s": round(72.0 * cells / (d32 / args.steps * 1e-3) / 1e9, 1),
+            "vs_fp64": round(m32 / mlups, 3)}
+        log(f"FP32-shifted {N}^2: {m32:.0f} MLUPS ({m32 / mlups:.2f}x FP64)")
+        lat = cylinder_lattice(native.F64)
+        wc, dc = time_lattice(lat, 500, 50)
+        lat.close()
+        secondary["cylinder_fp64"] = {
+            "workload": "cylinder_channel_2048x512_fp64_ibb_regularized",
+            "MLUPS": round(2048 * 512 * 500 / wc / 1e6, 1),
+            "ms_per_step": round(wc / 500 * 1e3, 4)}
+        log(f"cylinder 2048x512 FP64: {secondary['cylinder_fp64']}")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.cpu_size)
+            log(f"cpu baseline: {cpu}")
+        except Exception as e:   # reported, not fatal: the GPU number stands
+            log(f"cpu baseline failed: {e}")
+
+    traffic = load_traffic(f"tgv{N}_fp64")
+    out = {
+        "metric": "MLUPS (FP64 fused D2Q9 BGK step) + % HBM roofline",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green vortex generated on the device)",
+        "config": {"workload": f"taylor_green_periodic_{N}x{N}_fp64", "Nx": N, "Ny": N,
+                   "tau": 0.8, "u0": 0.01, "lattice": "D2Q9 BGK", "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+        "secondary": secondary,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

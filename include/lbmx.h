@@ -23,6 +23,12 @@
 
 #include <stdint.h>
 
+#if defined(__GNUC__)
+#define LBMX_API __attribute__((visibility("default")))
+#else
+#define LBMX_API
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -76,36 +82,42 @@ typedef struct {
     unsigned flags;       /* LBMX_EXACT_STATE */
 } lbmx_config;
 
-const char *lbmx_last_error(void);
+LBMX_API const char *lbmx_last_error(void);
 /* Name of the device the library runs on (for logs), or "" when none. */
-const char *lbmx_device_name(void);
+LBMX_API const char *lbmx_device_name(void);
 
 /* Replaces main_lbm's per-call arrays (lbm_core.h:6-31) by a resident lattice. */
-int lbmx_create(const lbmx_config *cfg, lbmx_lattice **out);
-void lbmx_destroy(lbmx_lattice *lat);
+LBMX_API int lbmx_create(const lbmx_config *cfg, lbmx_lattice **out);
+LBMX_API void lbmx_destroy(lbmx_lattice *lat);
 
 /* Geometry: solid mask [Ny][Nx] (required), omega_eff [Ny][Nx] (required),
  * phi [Ny][Nx] signed distance (required when use_ibb).  Copied; the link list
  * and node classes are compiled here.  Replaces the solid_mask/omega_eff/phi
  * arguments of main_lbm (lbm_core.h:19-20, 29). */
-int lbmx_set_geometry(lbmx_lattice *lat, const int *solid_mask, const double *omega,
+LBMX_API int lbmx_set_geometry(lbmx_lattice *lat, const int *solid_mask, const double *omega,
                       const double *phi);
+
+/* All-fluid geometry with one relaxation rate (no host arrays). */
+LBMX_API int lbmx_set_uniform(lbmx_lattice *lat, double omega);
 
 /* Boundary table in application order (main_lbm.c:49-58): types[b] is an
  * LBMX_BC_* code, index[b] the column (inlet/outlet) or row (top/bottom). */
-int lbmx_set_boundaries(lbmx_lattice *lat, const int *types, const int *index, int nbc);
+LBMX_API int lbmx_set_boundaries(lbmx_lattice *lat, const int *types, const int *index, int nbc);
+
+/* Body force of the following steps (main_lbm's Fx, Fy may change per call). */
+LBMX_API int lbmx_set_force(lbmx_lattice *lat, double Fx, double Fy);
 
 /* Values of boundary b for the next step: val1/val2 are host arrays of the edge
  * length (Ny for inlet/outlet, Nx for top/bottom) or NULL when the boundary does
  * not read them (Boundary.val1/val2, lbm_types.h:22-27). */
-int lbmx_set_bc_values(lbmx_lattice *lat, int b, const double *val1, const double *val2);
+LBMX_API int lbmx_set_bc_values(lbmx_lattice *lat, int b, const double *val1, const double *val2);
 
 /* Starts a run from host state in the reference layout: f is what the caller's
  * `f` holds before main_lbm, f_new what its `f_new` holds (its entries that the
  * step does not overwrite survive, as in streaming.c:36-40), and rho/u/v the
  * moment arrays the first collision reads (collision.c:24-32).  f_new may be
  * NULL (then it is a copy of f). */
-int lbmx_upload(lbmx_lattice *lat, const double *f, const double *f_new, const double *rho,
+LBMX_API int lbmx_upload(lbmx_lattice *lat, const double *f, const double *f_new, const double *rho,
                 const double *u, const double *v);
 
 /* Starts a run from a Taylor-Green vortex generated on the device (no host
@@ -113,38 +125,81 @@ int lbmx_upload(lbmx_lattice *lat, const double *f, const double *f_new, const d
  * rho = 1 - (3 u0^2/4)(cos 2kx x + (kx/ky)^2 cos 2ky y), both buffers at
  * equilibrium, moment arrays = (rho, u, v).  For benchmarks and large runs
  * (the configuration of cases/taylorgreen/taylorgreen.c with periodic axes). */
-int lbmx_init_taylor_green(lbmx_lattice *lat, double u0, double kx, double ky);
+LBMX_API int lbmx_init_taylor_green(lbmx_lattice *lat, double u0, double kx, double ky);
 
 /* Advances nsteps time steps; each is main_lbm followed by the driver's f/f_new
  * swap.  Asynchronous on the lattice's stream. */
-int lbmx_step(lbmx_lattice *lat, int nsteps);
+LBMX_API int lbmx_step(lbmx_lattice *lat, int nsteps);
+
+/* Time-domain impedance boundary (TDIBC) driven on the device every step, the
+ * scheme of cases/nit/nit.c:13-47 and :275-303: before each step the wall
+ * pressure p' = cs*cs*(rho - rho0)*cp is taken from the lagged density of the
+ * probe line, an explicit-Euler auxiliary ODE (N=2 real poles, N=2 complex pole
+ * pairs, updated in place as nit.c:31 does) gives the wall velocity
+ * v = Zinf p' - A.phi - 2(B.psi + C.chi), and value = gain * (v / cu) is written
+ * into the driven boundary's val1 or val2 at every active edge node (0 elsewhere).
+ * Lumped mode (per_node = 0): one ODE on the probe line's mean density (nit).
+ * Per-node mode: one ODE per edge node on the density of its probe-line node.
+ * All ODE state is FP64 on the device; the history records, per step, the
+ * (t, p', v) row nit.c:273 prints (values of the previous step, as it prints them). */
+typedef struct {
+    int bc;                 /* boundary-table entry whose values are driven */
+    int which;              /* 1: val1, 2: val2 */
+    int probe_is_column;    /* probe line is column probe_index (else row probe_index) */
+    int probe_index;
+    int per_node;
+    int record_node;        /* per-node mode: the edge node whose (p', v) is recorded */
+    int capacity;           /* history rows kept (steps) */
+    double rho0, cs, cp, dt, cu, gain;
+    double Zinf, A[2], lambdas[2], B[2], C[2], alpha[2], beta[2];
+    const int *active;      /* edge-length flags (NULL: all active) */
+} lbmx_tdibc;
+
+LBMX_API int lbmx_set_tdibc(lbmx_lattice *lat, const lbmx_tdibc *cfg);
+/* Copies min(rows, steps recorded) history rows (t, p', v) into rec[rows][3];
+ * returns the number of rows copied. */
+LBMX_API int lbmx_tdibc_history(lbmx_lattice *lat, double *rec, int rows);
+
+/* Density probes: every step records, on the device, the density at n nodes as
+ * the reference's rho array holds it at the start of that step (the previous
+ * step's compute_macroscopic_fields output).  capacity = steps kept. */
+LBMX_API int lbmx_set_probes(lbmx_lattice *lat, const int *x, const int *y, int n, int capacity);
+/* Copies min(rows, steps recorded) rows of n densities into rec[rows][n];
+ * returns the number of rows copied. */
+LBMX_API int lbmx_probe_history(lbmx_lattice *lat, double *rec, int rows);
+
+/* lbmx_step(nsteps) bracketed by HIP events recorded on the lattice's stream;
+ * *ms receives the device time between them (blocks until done). */
+LBMX_API int lbmx_time_steps(lbmx_lattice *lat, int nsteps, double *ms);
 
 /* Momentum-exchange force of the last step, as main_lbm returns it through
  * (L, D) (main_lbm.c:43): D = x-component, L = y-component. */
-int lbmx_forces(lbmx_lattice *lat, double *L, double *D);
+LBMX_API int lbmx_forces(lbmx_lattice *lat, double *L, double *D);
 
 /* Moments of the current state exactly as compute_macroscopic_fields leaves
  * them after the last step (NaN on solid nodes).  Any pointer may be NULL. */
-int lbmx_moments(lbmx_lattice *lat, double *rho, double *u, double *v);
+LBMX_API int lbmx_moments(lbmx_lattice *lat, double *rho, double *u, double *v);
 
 /* Current populations: f receives what the driver's `f` points to after the
- * last step and swap.  f_new (may be NULL) receives the other buffer; it is the
- * reference's content only for lattices created with LBMX_EXACT_STATE. */
-int lbmx_download(lbmx_lattice *lat, double *f, double *f_new);
+ * last step and swap (the newest state); f_new (may be NULL) what its `f_new`
+ * points to (the post-collision field of the last step, stale entries included).
+ * Both are the reference's exact buffer contents; without LBMX_EXACT_STATE the
+ * second one is completed on the device by this call. */
+LBMX_API int lbmx_download(lbmx_lattice *lat, double *f, double *f_new);
 
 /* Blocks until the lattice's queued work is done. */
-int lbmx_synchronize(lbmx_lattice *lat);
+LBMX_API int lbmx_synchronize(lbmx_lattice *lat);
 
 /* Steps run since lbmx_upload, and device bytes held by the lattice. */
-int64_t lbmx_steps_done(const lbmx_lattice *lat);
-int64_t lbmx_device_bytes(const lbmx_lattice *lat);
+LBMX_API int64_t lbmx_steps_done(const lbmx_lattice *lat);
+LBMX_API int64_t lbmx_device_bytes(const lbmx_lattice *lat);
 
 /* Mei consistent initialization on the device (mei_initialization.c:7-142):
  * same arguments and in/out contract as the reference function with the
  * lattice tables implied.  f/f_new are read as the starting buffers and
  * written back; rho/u/v receive the reference's outputs.  Returns the number
  * of iterations run (>= 1) or a negative error code. */
-int lbmx_mei_initialization(int Nx, int Ny, double *f, double *f_new, const double *rho_in,
+LBMX_API int lbmx_mei_initialization(int Nx, int Ny, double *f, double *f_new, const double *rho_in,
                             const double *U_in, const double *V_in, double *rho, double *u,
                             double *v, const double *omega, int per_x, int per_y,
                             int init_iter_max, double tol_rho, const int *solid_mask);
@@ -153,20 +208,30 @@ int lbmx_mei_initialization(int Nx, int Ny, double *f, double *f_new, const doub
  * (lbm_core.h:33-92, lbm_bc.h:4-355).  Each uploads its host arrays, runs the
  * stage on the device with the reference's semantics and writes its outputs
  * back.  bc_type is an LBMX_BC_* code. */
-int lbmx_stage_collision(int Nx, int Ny, double *f, const double *rho, const double *u,
+LBMX_API int lbmx_stage_collision(int Nx, int Ny, double *f, const double *rho, const double *u,
                          const double *v, const double *omega, const int *mask, double Fx,
                          double Fy);
-int lbmx_stage_streaming(int Nx, int Ny, const double *f, double *f_new, const int *mask,
+LBMX_API int lbmx_stage_streaming(int Nx, int Ny, const double *f, double *f_new, const int *mask,
                          int per_x, int per_y);
-int lbmx_stage_bounce_back(int Nx, int Ny, const double *f, double *f_new, const int *mask,
+LBMX_API int lbmx_stage_bounce_back(int Nx, int Ny, const double *f, double *f_new, const int *mask,
                            int per_x, int per_y, int use_ibb, const double *phi);
-int lbmx_stage_force(int Nx, int Ny, const double *f, const double *f_new, const int *mask,
+LBMX_API int lbmx_stage_force(int Nx, int Ny, const double *f, const double *f_new, const int *mask,
                      int per_x, int per_y, double *Fx, double *Fy);
-int lbmx_stage_moments(int Nx, int Ny, const double *f_new, const int *mask, double *rho,
+LBMX_API int lbmx_stage_moments(int Nx, int Ny, const double *f_new, const int *mask, double *rho,
                        double *u, double *v, double Fx, double Fy);
-int lbmx_stage_bc(int Nx, int Ny, int bc_type, int index, const double *val1,
+LBMX_API int lbmx_stage_bc(int Nx, int Ny, int bc_type, int index, const double *val1,
                   const double *val2, const double *f, double *f_new, const double *rho,
                   const double *u, const double *v, const int *mask);
+
+/* Frees the device lattice the stand-alone stages cache between calls. */
+LBMX_API void lbmx_stage_release(void);
+
+/* Extensions of the drop-in API (no counterpart in the reference):
+ * lbm_hip_sync writes the device state of the running main_lbm sequence back
+ * into the caller's f/f_new (for callers that read populations mid-run);
+ * lbm_hip_release ends that sequence and frees its device memory. */
+LBMX_API void lbm_hip_sync(void);
+LBMX_API void lbm_hip_release(void);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,7 @@ CC = os.environ.get("CC", "gcc")
 # roundings (its x86-64 build has no FMA); FP32 code opts back in locally with
 # `#pragma clang fp contract(fast)`.  Packed-FP32 instructions are disabled.
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fvisibility=hidden",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", "-Wno-unused-value", "-Wno-unused-function",
     "-Wno-unused-variable",

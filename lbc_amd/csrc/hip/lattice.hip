@@ -195,6 +195,9 @@ extern "C" void lbmx_destroy(lbmx_lattice *L)
     hipFree(L->R);
     hipFree(L->U);
     hipFree(L->V);
+    hipFree(L->Ro);
+    hipFree(L->Uo);
+    hipFree(L->Vo);
     hipFree(L->cls);
     hipFree(L->lut);
     hipFree(L->om_arr);
@@ -205,6 +208,12 @@ extern "C" void lbmx_destroy(lbmx_lattice *L)
     for (double *p : L->val1) hipFree(p);
     for (double *p : L->val2) hipFree(p);
     hipFree(L->staging);
+    hipFree(L->tdibc.st);
+    hipFree(L->tdibc.last);
+    hipFree(L->tdibc.active);
+    hipFree(L->tdibc.hist);
+    hipFree(L->probes.nodes);
+    hipFree(L->probes.hist);
     if (L->force_host) hipHostFree(L->force_host);
     if (L->stream) hipStreamDestroy(L->stream);
     delete L;
@@ -351,6 +360,28 @@ extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double 
     // stale-edge enumeration (k_edges)
     L->edge_rows = py ? 0 : 2 * Nx;
     L->edge_cols = px ? 0 : (py ? 2 * Ny : 2 * (Ny - 2));
+    L->has_geometry = 1;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_set_uniform(lbmx_lattice *L, double omega)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(L->stream));
+    hipFree(L->cls); L->cls = nullptr;
+    hipFree(L->lut); L->lut = nullptr;
+    hipFree(L->om_arr); L->om_arr = nullptr;
+    hipFree(L->links); L->links = nullptr;
+    hipFree(L->partials); L->partials = nullptr;
+    L->geom = 0;
+    L->om_uniform = omega;
+    L->nsolid = 0;
+    L->nlinks = 0;
+    L->npartials = 0;
+    HIPCHK(hipMemsetAsync(L->force_dev, 0, 2 * sizeof(double), L->stream));
+    L->edge_rows = L->cfg.per_y ? 0 : 2 * L->Nx;
+    L->edge_cols = L->cfg.per_x ? 0 : (L->cfg.per_y ? 2 * L->Ny : 2 * (L->Ny - 2));
     L->has_geometry = 1;
     HIPCHK(hipStreamSynchronize(L->stream));
     return LBMX_OK;
@@ -545,11 +576,25 @@ extern "C" int lbmx_upload(lbmx_lattice *L, const double *f, const double *f_new
     if (!rc) rc = lbmx_put_field(L, L->V, v);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(L->stream));
+    lbmx_reset_run(L);
+    return LBMX_OK;
+}
+
+void lbmx_reset_run(lbmx_lattice *L)
+{
     L->lag_arrays = 1;
     L->has_state = 1;
     L->steps = 0;
     L->Fx_lag = L->cfg.Fx;
     L->Fy_lag = L->cfg.Fy;
+    L->other_is_ref = 1;
+}
+
+extern "C" int lbmx_set_force(lbmx_lattice *L, double Fx, double Fy)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    L->cfg.Fx = Fx;
+    L->cfg.Fy = Fy;
     return LBMX_OK;
 }
 
@@ -569,11 +614,25 @@ extern "C" int lbmx_init_taylor_green(lbmx_lattice *L, double u0, double kx, dou
                            (float *)L->pop[0], (float *)L->pop[1], F, g, u0, kx, ky);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(L->stream));
-    L->lag_arrays = 1;
-    L->has_state = 1;
-    L->steps = 0;
-    L->Fx_lag = L->cfg.Fx;
-    L->Fy_lag = L->cfg.Fy;
+    lbmx_reset_run(L);
+    return LBMX_OK;
+}
+
+// Completes the post-collision field in the non-current buffer (fast mode).
+int lbmx_materialize_other(lbmx_lattice *L)
+{
+    if (L->other_is_ref) return LBMX_OK;
+    const Geo g = lbmx_geo(L);
+    const Fields F{L->R, L->U, L->V};
+    dim3 grid((L->Nx + 255) / 256, L->Ny);
+    if (L->esize == 8)
+        hipLaunchKernelGGL(k_materialize<double>, grid, dim3(256), 0, L->stream,
+                           (double *)L->pop[L->cur ^ 1], g, L->last, F);
+    else
+        hipLaunchKernelGGL(k_materialize<float>, grid, dim3(256), 0, L->stream,
+                           (float *)L->pop[L->cur ^ 1], g, L->last, F);
+    HIPCHK(hipGetLastError());
+    L->other_is_ref = 1;
     return LBMX_OK;
 }
 
@@ -583,6 +642,7 @@ extern "C" int lbmx_download(lbmx_lattice *L, double *f, double *f_new)
     if (!L->has_state) return lbmx_fail(LBMX_ERR_STATE, "no state uploaded");
     int rc = LBMX_OK;
     if (f) rc = lbmx_get_pops(L, L->cur, f);
+    if (!rc && f_new) rc = lbmx_materialize_other(L);
     if (!rc && f_new) rc = lbmx_get_pops(L, L->cur ^ 1, f_new);
     return rc;
 }
@@ -604,6 +664,13 @@ StepFlags lbmx_flags(const lbmx_lattice *L)
 }
 
 template <typename T>
+static int launch_tdibc(lbmx_lattice *L, const T *S, const Geo &g, const StepFlags &s,
+                        const Fields &F);
+template <typename T>
+static int launch_probes(lbmx_lattice *L, const T *S, const Geo &g, const StepFlags &s,
+                         const Fields &F);
+
+template <typename T>
 static int launch_step(lbmx_lattice *L)
 {
     T *S = (T *)L->pop[L->cur];
@@ -611,6 +678,15 @@ static int launch_step(lbmx_lattice *L)
     const Geo g = lbmx_geo(L);
     const StepFlags s = lbmx_flags(L);
     const Fields F{L->R, L->U, L->V};
+
+    if (L->probes.n) {
+        int rc = launch_probes<T>(L, S, g, s, F);
+        if (rc) return rc;
+    }
+    if (L->tdibc.on) {
+        int rc = launch_tdibc<T>(L, S, g, s, F);
+        if (rc) return rc;
+    }
 
     BulkArgs ba;
     ba.g = g;
@@ -671,6 +747,8 @@ static int launch_step(lbmx_lattice *L)
         HIPCHK(hipGetLastError());
     }
     L->cur ^= 1;
+    L->last = s;
+    L->other_is_ref = s.exact;
     L->lag_arrays = 0;
     L->Fx_lag = L->cfg.Fx;
     L->Fy_lag = L->cfg.Fy;
@@ -692,6 +770,208 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
     return LBMX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// time-domain impedance boundary controller
+// ---------------------------------------------------------------------------
+static void tdibc_free(lbmx_lattice *L)
+{
+    hipFree(L->tdibc.st);
+    hipFree(L->tdibc.last);
+    hipFree(L->tdibc.active);
+    hipFree(L->tdibc.hist);
+    L->tdibc.st = nullptr;
+    L->tdibc.last = nullptr;
+    L->tdibc.active = nullptr;
+    L->tdibc.hist = nullptr;
+    L->tdibc.on = 0;
+}
+
+extern "C" int lbmx_set_tdibc(lbmx_lattice *L, const lbmx_tdibc *c)
+{
+    if (!L || !c) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (c->bc < 0 || c->bc >= (int)L->types.size())
+        return lbmx_fail(LBMX_ERR_ARG, "tdibc: boundary %d not in the table", c->bc);
+    const int t = L->types[c->bc];
+    if (c->which < 1 || c->which > lbmx_bc_nvals(t))
+        return lbmx_fail(LBMX_ERR_ARG, "tdibc: boundary type %d has no val%d", t, c->which);
+    const int col = lbmx_bc_on_column(t);
+    if ((c->probe_is_column != 0) != (col != 0))
+        return lbmx_fail(LBMX_ERR_ARG, "tdibc: probe line must run along the boundary");
+    const int nedge = col ? L->Ny : L->Nx;
+    const int span = col ? L->Nx : L->Ny;
+    if (c->probe_index < 0 || c->probe_index >= span)
+        return lbmx_fail(LBMX_ERR_ARG, "tdibc: probe index %d out of range", c->probe_index);
+    if (c->capacity < 0) return lbmx_fail(LBMX_ERR_ARG, "tdibc: negative capacity");
+    HIPCHK(hipStreamSynchronize(L->stream));
+    tdibc_free(L);
+    TdibcParams p{};
+    p.which = c->which;
+    p.probe_is_column = c->probe_is_column ? 1 : 0;
+    p.probe_index = c->probe_index;
+    p.per_node = c->per_node ? 1 : 0;
+    p.record_node = c->record_node;
+    p.capacity = c->capacity;
+    p.nedge = nedge;
+    p.rho0 = c->rho0;
+    p.cs = c->cs;
+    p.cp = c->cp;
+    p.dt = c->dt;
+    p.cu = c->cu;
+    p.gain = c->gain;
+    p.Zinf = c->Zinf;
+    for (int i = 0; i < 2; i++) {
+        p.A[i] = c->A[i];
+        p.lambdas[i] = c->lambdas[i];
+        p.B[i] = c->B[i];
+        p.C[i] = c->C[i];
+        p.alpha[i] = c->alpha[i];
+        p.beta[i] = c->beta[i];
+    }
+    const int nst = p.per_node ? nedge : 1;
+    HIPCHK(hipMalloc((void **)&L->tdibc.st, nst * sizeof(TdibcState)));
+    HIPCHK(hipMemset(L->tdibc.st, 0, nst * sizeof(TdibcState)));
+    HIPCHK(hipMalloc((void **)&L->tdibc.last, 2 * sizeof(double)));
+    HIPCHK(hipMemset(L->tdibc.last, 0, 2 * sizeof(double)));
+    if (c->active) {
+        HIPCHK(hipMalloc((void **)&L->tdibc.active, nedge * sizeof(int)));
+        HIPCHK(hipMemcpy(L->tdibc.active, c->active, nedge * sizeof(int),
+                         hipMemcpyHostToDevice));
+    }
+    if (p.capacity > 0) {
+        HIPCHK(hipMalloc((void **)&L->tdibc.hist, (size_t)p.capacity * 3 * sizeof(double)));
+        HIPCHK(hipMemset(L->tdibc.hist, 0, (size_t)p.capacity * 3 * sizeof(double)));
+    }
+    L->tdibc.p = p;
+    L->tdibc.bc = c->bc;
+    L->tdibc.step = 0;
+    L->tdibc.on = 1;
+    // the controller provides this boundary's driven values from now on
+    L->vals_set[c->bc] = 1;
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_tdibc_history(lbmx_lattice *L, double *rec, int rows)
+{
+    if (!L || !rec) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!L->tdibc.on) return lbmx_fail(LBMX_ERR_STATE, "no tdibc controller");
+    long long n = L->tdibc.step < L->tdibc.p.capacity ? L->tdibc.step : L->tdibc.p.capacity;
+    if (n > rows) n = rows;
+    if (n > 0)
+        HIPCHK(hipMemcpyAsync(rec, L->tdibc.hist, (size_t)n * 3 * sizeof(double),
+                              hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return (int)n;
+}
+
+template <typename T>
+static int launch_tdibc(lbmx_lattice *L, const T *S, const Geo &g, const StepFlags &s,
+                        const Fields &F)
+{
+    TdibcArgs a;
+    a.g = g;
+    a.s = s;
+    a.F = F;
+    a.p = L->tdibc.p;
+    a.st = L->tdibc.st;
+    a.last = L->tdibc.last;
+    a.active = L->tdibc.active;
+    a.val = (L->tdibc.p.which == 1) ? L->val1[L->tdibc.bc] : L->val2[L->tdibc.bc];
+    a.hist = L->tdibc.hist;
+    a.step = L->tdibc.step;
+    if (a.p.per_node)
+        hipLaunchKernelGGL(k_tdibc_nodes<T>, dim3((a.p.nedge + 255) / 256), dim3(256), 0,
+                           L->stream, S, a);
+    else
+        hipLaunchKernelGGL(k_tdibc_lumped<T>, dim3(1), dim3(kTdibcThreads), 0, L->stream, S, a);
+    HIPCHK(hipGetLastError());
+    L->tdibc.step++;
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_set_probes(lbmx_lattice *L, const int *x, const int *y, int n, int capacity)
+{
+    if (!L || n < 0 || capacity < 0 || (n > 0 && (!x || !y)))
+        return lbmx_fail(LBMX_ERR_ARG, "bad probe list");
+    std::vector<long long> nodes(n);
+    for (int i = 0; i < n; i++) {
+        if (x[i] < 0 || x[i] >= L->Nx || y[i] < 0 || y[i] >= L->Ny)
+            return lbmx_fail(LBMX_ERR_ARG, "probe %d at (%d, %d) outside the lattice", i, x[i],
+                             y[i]);
+        nodes[i] = (long long)y[i] * L->pitch + x[i];
+    }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    hipFree(L->probes.nodes);
+    hipFree(L->probes.hist);
+    L->probes.nodes = nullptr;
+    L->probes.hist = nullptr;
+    L->probes.n = 0;
+    if (n == 0 || capacity == 0) return LBMX_OK;
+    HIPCHK(hipMalloc((void **)&L->probes.nodes, n * sizeof(long long)));
+    HIPCHK(hipMemcpy(L->probes.nodes, nodes.data(), n * sizeof(long long),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void **)&L->probes.hist, (size_t)n * capacity * sizeof(double)));
+    HIPCHK(hipMemset(L->probes.hist, 0, (size_t)n * capacity * sizeof(double)));
+    L->probes.n = n;
+    L->probes.capacity = capacity;
+    L->probes.step = 0;
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_probe_history(lbmx_lattice *L, double *rec, int rows)
+{
+    if (!L || !rec) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!L->probes.n) return lbmx_fail(LBMX_ERR_STATE, "no probes");
+    long long m = L->probes.step < L->probes.capacity ? L->probes.step : L->probes.capacity;
+    if (m > rows) m = rows;
+    if (m > 0)
+        HIPCHK(hipMemcpyAsync(rec, L->probes.hist, (size_t)m * L->probes.n * sizeof(double),
+                              hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return (int)m;
+}
+
+template <typename T>
+static int launch_probes(lbmx_lattice *L, const T *S, const Geo &g, const StepFlags &s,
+                         const Fields &F)
+{
+    ProbeArgs a;
+    a.g = g;
+    a.s = s;
+    a.F = F;
+    a.nodes = L->probes.nodes;
+    a.n = L->probes.n;
+    a.capacity = L->probes.capacity;
+    a.hist = L->probes.hist;
+    a.step = L->probes.step;
+    hipLaunchKernelGGL(k_probes<T>, dim3((a.n + 255) / 256), dim3(256), 0, L->stream, S, a);
+    HIPCHK(hipGetLastError());
+    L->probes.step++;
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_time_steps(lbmx_lattice *L, int nsteps, double *ms)
+{
+    if (!L || !ms) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, L->stream));
+    int rc = lbmx_step(L, nsteps);
+    if (rc) {
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        return rc;
+    }
+    HIPCHK(hipEventRecord(b, L->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms = t;
+    return LBMX_OK;
+}
+
 extern "C" int lbmx_forces(lbmx_lattice *L, double *Lift, double *Drag)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
@@ -705,9 +985,17 @@ extern "C" int lbmx_forces(lbmx_lattice *L, double *Lift, double *Drag)
 
 int lbmx_launch_moments(lbmx_lattice *L)
 {
+    if (!L->Ro) {
+        const size_t fld = (size_t)L->Ny * L->pitch * sizeof(double);
+        if (hipMalloc((void **)&L->Ro, fld) != hipSuccess ||
+            hipMalloc((void **)&L->Uo, fld) != hipSuccess ||
+            hipMalloc((void **)&L->Vo, fld) != hipSuccess)
+            return lbmx_fail(LBMX_ERR_NOMEM, "moment output arrays (%zu bytes)", 3 * fld);
+        L->bytes += 3 * fld;
+    }
     MomArgs m;
     m.g = lbmx_geo(L);
-    m.F = Fields{L->R, L->U, L->V};
+    m.F = Fields{L->Ro, L->Uo, L->Vo};
     m.Fx = L->cfg.Fx;
     m.Fy = L->cfg.Fy;
     dim3 grid((L->Nx + 255) / 256, L->Ny);
@@ -725,14 +1013,20 @@ extern "C" int lbmx_moments(lbmx_lattice *L, double *rho, double *u, double *v)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
     if (!L->has_state) return lbmx_fail(LBMX_ERR_STATE, "no state uploaded");
-    if (!L->lag_arrays) {
+    // before the first step the arrays are the uploaded ones, as the reference's
+    // would be; afterwards compute_macroscopic_fields' result of the last step
+    const double *r = L->R, *uu = L->U, *vv = L->V;
+    if (L->steps > 0) {
         int rc = lbmx_launch_moments(L);
         if (rc) return rc;
+        r = L->Ro;
+        uu = L->Uo;
+        vv = L->Vo;
     }
     int rc = LBMX_OK;
-    if (rho) rc = lbmx_get_field(L, rho, L->R);
-    if (!rc && u) rc = lbmx_get_field(L, u, L->U);
-    if (!rc && v) rc = lbmx_get_field(L, v, L->V);
+    if (rho) rc = lbmx_get_field(L, rho, r);
+    if (!rc && u) rc = lbmx_get_field(L, u, uu);
+    if (!rc && v) rc = lbmx_get_field(L, v, vv);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(L->stream));
     return LBMX_OK;

@@ -7,6 +7,7 @@
 
 #include "lbmx.h"
 #include "step_kernels.h"
+#include "tdibc_kernels.h"
 
 constexpr int kQ = 9;
 
@@ -56,6 +57,30 @@ struct lbmx_lattice {
 
     int has_geometry = 0, has_state = 0, lag_arrays = 0;
     double Fx_lag = 0.0, Fy_lag = 0.0;
+    // output moments (compute_macroscopic_fields' result); R/U/V keep the lagged
+    // moments the last step read, which k_materialize needs
+    double *Ro = nullptr, *Uo = nullptr, *Vo = nullptr;
+    int other_is_ref = 0;        // the non-current buffer holds the reference's f
+    lbmx::StepFlags last{};      // flags of the last step run
+
+    // time-domain impedance boundary controller (lbmx_set_tdibc)
+    struct {
+        int on = 0, bc = -1;
+        lbmx::TdibcParams p{};
+        lbmx::TdibcState *st = nullptr;
+        double *last = nullptr;
+        int *active = nullptr;
+        double *hist = nullptr;
+        long long step = 0;
+    } tdibc;
+
+    // density probes (lbmx_set_probes)
+    struct {
+        int n = 0, capacity = 0;
+        long long *nodes = nullptr;
+        double *hist = nullptr;
+        long long step = 0;
+    } probes;
     int64_t steps = 0;
     int ncu = 0, nxcd = 8, bulk_grid = 0;
     int64_t bytes = 0;
@@ -70,3 +95,5 @@ int lbmx_get_pops(lbmx_lattice *L, int which, double *aos);
 int lbmx_put_field(lbmx_lattice *L, double *dev, const double *host);
 int lbmx_get_field(lbmx_lattice *L, double *host, const double *dev);
 int lbmx_launch_moments(lbmx_lattice *L);
+int lbmx_materialize_other(lbmx_lattice *L);
+void lbmx_reset_run(lbmx_lattice *L);

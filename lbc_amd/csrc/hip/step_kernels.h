@@ -702,6 +702,9 @@ __global__ void __launch_bounds__(256) k_edges(T *__restrict__ S, EdgeArgs a)
     }
     const size_t n = (size_t)y * g.pitch + x;
     if (solid_at(g, n)) return;
+    // the lagged moments this step used at n are kept in R/U/V so that the full
+    // post-collision field can still be rebuilt later (k_materialize)
+    const Moments m = lagged_at(S, g, a.F, a.s, n);
     double st[Q];
     star_node(S, g, a.F, a.s, n, st);
 #pragma unroll
@@ -711,6 +714,41 @@ __global__ void __launch_bounds__(256) k_edges(T *__restrict__ S, EdgeArgs a)
                          (!g.per_y && (ys < 0 || ys >= g.Ny));
         if (out) pop_put(S, k * g.plane + n, k, st[k], g.rho0);
     }
+    a.F.R[n] = m.rho;
+    a.F.U[n] = m.ux;
+    a.F.V[n] = m.uy;
+}
+
+// ---- k_materialize: the reference's post-collision buffer, on demand ----------------
+// After a fast-mode step the buffer that was S holds the pre-collision state
+// except in the stale slots of edge nodes (already post-collision, k_edges).  This
+// kernel finishes the collision in place so the buffer equals the reference's
+// `f` after main_lbm: lagged moments from R/U/V where they were kept (edge nodes,
+// or every node when the step read the uploaded arrays), else from the buffer.
+template <typename T>
+__global__ void __launch_bounds__(256) k_materialize(T *__restrict__ S, Geo g, StepFlags s,
+                                                     Fields F)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= g.Nx) return;
+    const size_t n = (size_t)y * g.pitch + x;
+    if (solid_at(g, n)) return;
+    bool stale[Q];
+    bool edge = false;
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const int xs = x - D2Q9::cx[k], ys = y - D2Q9::cy[k];
+        stale[k] = (!g.per_x && (xs < 0 || xs >= g.Nx)) || (!g.per_y && (ys < 0 || ys >= g.Ny));
+        edge = edge || stale[k];
+    }
+    StepFlags t = s;
+    t.lag_arrays = s.lag_arrays || edge;   // R/U/V hold the lagged moments here
+    t.exact = 0;
+    double st[Q];
+    star_node(S, g, F, t, n, st);
+#pragma unroll
+    for (int k = 0; k < Q; k++)
+        if (!stale[k]) pop_put(S, k * g.plane + n, k, st[k], g.rho0);
 }
 
 // ---- k_moments: compute_macroscopic_fields.c:19-48 --------------------------------

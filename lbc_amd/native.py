@@ -30,6 +30,15 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 
+class Tdibc(C.Structure):
+    """lbmx_tdibc (include/lbmx.h)."""
+    _fields_ = [(n, C.c_int) for n in ("bc", "which", "probe_is_column", "probe_index",
+                                       "per_node", "record_node", "capacity")] + \
+               [(n, C.c_double) for n in ("rho0", "cs", "cp", "dt", "cu", "gain", "Zinf")] + \
+               [(n, C.c_double * 2) for n in ("A", "lambdas", "B", "C", "alpha", "beta")] + \
+               [("active", C.POINTER(C.c_int))]
+
+
 class Config(C.Structure):
     _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("precision", C.c_int),
                 ("per_x", C.c_int), ("per_y", C.c_int), ("use_ibb", C.c_int),
@@ -71,6 +80,13 @@ def _bind():
     L.lbmx_upload.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
     L.lbmx_step.argtypes = [vp, i]
     L.lbmx_init_taylor_green.argtypes = [vp, d, d, d]
+    L.lbmx_set_uniform.argtypes = [vp, d]
+    L.lbmx_set_force.argtypes = [vp, d, d]
+    L.lbmx_time_steps.argtypes = [vp, i, _dp]
+    L.lbmx_set_tdibc.argtypes = [vp, C.POINTER(Tdibc)]
+    L.lbmx_tdibc_history.argtypes = [vp, _dp, i]
+    L.lbmx_set_probes.argtypes = [vp, _ip, _ip, i, i]
+    L.lbmx_probe_history.argtypes = [vp, _dp, i]
     L.lbmx_forces.argtypes = [vp, _dp, _dp]
     L.lbmx_moments.argtypes = [vp, _dp, _dp, _dp]
     L.lbmx_download.argtypes = [vp, _dp, _dp]
@@ -144,6 +160,56 @@ class Lattice:
     def upload(self, f, f_new, rho, u, v):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (f, f_new, rho, u, v)]
         check(self._L.lbmx_upload(self.h, *[_d(a) for a in arrs]))
+
+    def set_uniform(self, omega):
+        check(self._L.lbmx_set_uniform(self.h, float(omega)))
+
+    def set_force(self, Fx, Fy):
+        check(self._L.lbmx_set_force(self.h, float(Fx), float(Fy)))
+
+    def time_steps(self, n):
+        """Device milliseconds (HIP events on the lattice's stream) for n steps."""
+        ms = C.c_double()
+        check(self._L.lbmx_time_steps(self.h, int(n), C.byref(ms)))
+        return ms.value
+
+    def set_tdibc(self, bc, which, probe_is_column, probe_index, per_node, record_node,
+                  capacity, rho0, cs, cp, dt, cu, gain, Zinf, A, lambdas, B, C, alpha, beta,
+                  active=None):
+        t = Tdibc()
+        t.bc, t.which, t.probe_is_column, t.probe_index = bc, which, probe_is_column, probe_index
+        t.per_node, t.record_node, t.capacity = per_node, record_node, capacity
+        t.rho0, t.cs, t.cp, t.dt, t.cu, t.gain, t.Zinf = rho0, cs, cp, dt, cu, gain, Zinf
+        for name, val in (("A", A), ("lambdas", lambdas), ("B", B), ("C", C),
+                          ("alpha", alpha), ("beta", beta)):
+            arr = getattr(t, name)
+            arr[0], arr[1] = val
+        self._tdibc_active = None
+        if active is not None:
+            self._tdibc_active = np.ascontiguousarray(active, dtype=np.int32)
+            t.active = _i(self._tdibc_active)
+        check(self._L.lbmx_set_tdibc(self.h, C.byref(t)))
+        self._tdibc_capacity = capacity
+
+    def tdibc_history(self):
+        rec = np.zeros((max(1, self._tdibc_capacity), 3))
+        n = self._L.lbmx_tdibc_history(self.h, _d(rec), rec.shape[0])
+        if n < 0:
+            check(n)
+        return rec[:n]
+
+    def set_probes(self, xs, ys, capacity):
+        x = np.ascontiguousarray(xs, dtype=np.int32)
+        y = np.ascontiguousarray(ys, dtype=np.int32)
+        check(self._L.lbmx_set_probes(self.h, _i(x), _i(y), len(x), int(capacity)))
+        self._probe_shape = (int(capacity), len(x))
+
+    def probe_history(self):
+        rec = np.zeros(self._probe_shape)
+        n = self._L.lbmx_probe_history(self.h, _d(rec), rec.shape[0])
+        if n < 0:
+            check(n)
+        return rec[:n]
 
     def init_taylor_green(self, u0, kx, ky):
         check(self._L.lbmx_init_taylor_green(self.h, float(u0), float(kx), float(ky)))

@@ -68,7 +68,15 @@ class OrcTdibc(C.Structure):
 class OrcNitCfg(C.Structure):
     _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("nt", C.c_int), ("init_iter_max", C.c_int)] + \
                [(n, C.c_double) for n in ("dx", "dt", "cu", "cp", "cs", "rho0", "tau", "Ly",
-                                          "y1", "y2", "d", "tol_rho")] + [("n_orifices", C.c_int)]
+                                          "y1", "y2", "d", "tol_rho")] + \
+              [("n_orifices", C.c_int), ("variant", C.c_int)]
+
+
+class OrcLinerCfg(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("Nx", "Ny", "liner_x0", "liner_x1", "sponge", "probe_x",
+                                       "probe_y")] + \
+               [(n, C.c_double) for n in ("tau", "tau_sponge", "rho0", "amp", "x0",
+                                          "halfwidth", "gain")] + [("imp", OrcTdibc)]
 
 
 def _load_oracle():
@@ -97,6 +105,8 @@ def _load_oracle():
     lib.orc_nit_run.argtypes = [C.POINTER(OrcNitCfg), d, d, C.POINTER(OrcTdibc), i, _dp,
                                 _dp, _dp]
     lib.orc_nit_run.restype = i
+    lib.orc_liner_setup.argtypes = [C.POINTER(OrcLinerCfg), _dp, _dp, _dp, _dp, _dp, _ip]
+    lib.orc_liner_run.argtypes = [C.POINTER(OrcLinerCfg), i, _dp, _dp, _dp, _dp]
     return lib
 
 

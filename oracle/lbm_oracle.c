@@ -560,6 +560,7 @@ void orc_nit_config(double p_a, double f_exc, orc_nit_cfg *c)
     double c0 = 340.0, nu = 1.5e-5;
     int tot_cycles = 1 + 10 + 2;
     c->n_orifices = 2;
+    c->variant = 0;
     c->Ly = 0.009906; c->y1 = 0.002528; c->y2 = 0.007478; c->d = 0.00117;
     c->tau = 0.5005; c->rho0 = 1.184;
     c->cs = 1.0 / sqrt(3.0);
@@ -633,6 +634,7 @@ int orc_nit_run(const orc_nit_cfg *c, double p_a, double f_exc, orc_tdibc *imp, 
             int hole = (y >= c->y1 - c->d / 2 && y <= c->y1 + c->d / 2)
                      | (y >= c->y2 - c->d / 2 && y <= c->y2 + c->d / 2);
             vbc[j] = hole ? tv : 0.0;
+            if (c->variant == 1) vbc[j] = vac / c->cu;
         }
         orc_step(Nx, Ny, f, fn, rho, u, v, omega, mask, bcs, 3, 0, 1, 0.0, 0.0, &L, &D,
                  phi, 0);
@@ -643,4 +645,76 @@ int orc_nit_run(const orc_nit_cfg *c, double p_a, double f_exc, orc_tdibc *imp, 
     free(rho_in); free(U_in); free(V_in); free(rho); free(u); free(v); free(omega);
     free(phi); free(mask); free(fa); free(fb); free(r0); free(zero); free(vbc);
     return iters;
+}
+
+/* ---------------------------------------------------------------------------
+ * Acoustic-liner duct (see lbm_oracle.h).  Composes the restated reference
+ * routines: orc_step with the table {convective_inlet(0), convective_outlet(Nx-1),
+ * velocity_bottom_regularized(0)} and one orc_tdibc per liner node.
+ * ------------------------------------------------------------------------- */
+void orc_liner_setup(const orc_liner_cfg *c, double *f, double *rho, double *u, double *v,
+                     double *omega, int *mask)
+{
+    int Nx = c->Nx, Ny = c->Ny;
+    for (int j = 0; j < Ny; j++) {
+        for (int i = 0; i < Nx; i++) {
+            size_t n = NODE(i, j);
+            double xr = (i - c->x0) / c->halfwidth;
+            rho[n] = c->rho0 + c->amp * exp(-log(2.0) * xr * xr);
+            u[n] = 0.0;
+            v[n] = 0.0;
+            int wall = (j == Ny - 1) || (j == 0 && (i < c->liner_x0 || i > c->liner_x1));
+            mask[n] = wall;
+            double tau = c->tau;
+            int d = i < Nx - 1 - i ? i : Nx - 1 - i;   /* distance to the nearer end */
+            if (c->sponge > 0 && d < c->sponge)
+                tau = c->tau_sponge + (c->tau - c->tau_sponge) * ((double)d / c->sponge);
+            omega[n] = 1.0 / tau;
+            for (int k = 0; k < Q; k++) f[n * Q + k] = feq_c(k, rho[n], 0.0, 0.0, 0.0);
+        }
+    }
+}
+
+void orc_liner_run(const orc_liner_cfg *c, int nt, double *probe_hist, double *wall_p,
+                   double *f_out, double *rho_out)
+{
+    int Nx = c->Nx, Ny = c->Ny;
+    size_t nn = (size_t)Nx * Ny;
+    double *fa = malloc(nn * Q * 8), *fb = malloc(nn * Q * 8);
+    double *rho = malloc(nn * 8), *u = malloc(nn * 8), *v = malloc(nn * 8);
+    double *omega = malloc(nn * 8), *phi = calloc(nn, 8);
+    int *mask = malloc(nn * sizeof(int));
+    orc_liner_setup(c, fa, rho, u, v, omega, mask);
+    memcpy(fb, fa, nn * Q * 8);
+
+    double *zero = calloc(Ny > Nx ? Ny : Nx, 8), *vbc = calloc(Nx, 8);
+    int nl = c->liner_x1 - c->liner_x0 + 1;
+    orc_tdibc *st = malloc(nl * sizeof(orc_tdibc));
+    for (int l = 0; l < nl; l++) st[l] = c->imp;
+    orc_bc bcs[3] = {
+        {ORC_BC_CONVECTIVE_INLET, 0, NULL, NULL},
+        {ORC_BC_CONVECTIVE_OUTLET, Nx - 1, NULL, NULL},
+        {ORC_BC_VELOCITY_BOTTOM_REG, 0, zero, vbc},
+    };
+    const double cs = 1.0 / sqrt(3.0);
+    double L = 0.0, D = 0.0;
+    double *f = fa, *fn = fb;
+    for (int it = 0; it < nt; it++) {
+        if (probe_hist) probe_hist[it] = rho[NODE(c->probe_x, c->probe_y)];
+        for (int l = 0; l < nl; l++) {
+            int i = c->liner_x0 + l;
+            double p = cs * cs * (rho[NODE(i, 0)] - c->rho0) * 1.0;
+            double va = orc_tdibc_velocity(&st[l], p, 1.0);
+            vbc[i] = c->gain * (va / 1.0);
+        }
+        orc_step(Nx, Ny, f, fn, rho, u, v, omega, mask, bcs, 3, 0, 0, 0.0, 0.0, &L, &D, phi, 0);
+        double *t = f; f = fn; fn = t;
+    }
+    if (wall_p)
+        for (int l = 0; l < nl; l++)
+            wall_p[l] = cs * cs * (rho[NODE(c->liner_x0 + l, 0)] - c->rho0) * 1.0;
+    if (f_out) memcpy(f_out, f, nn * Q * 8);
+    if (rho_out) memcpy(rho_out, rho, nn * 8);
+    free(fa); free(fb); free(rho); free(u); free(v); free(omega); free(phi); free(mask);
+    free(zero); free(vbc); free(st);
 }

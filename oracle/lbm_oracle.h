@@ -112,10 +112,34 @@ typedef struct {
     int Nx, Ny, nt, init_iter_max;
     double dx, dt, cu, cp, cs, rho0, tau, Ly, y1, y2, d, tol_rho;
     int n_orifices;
+    /* 0: orifice rows get the area-scaled velocity (nit.c:286-300, the current
+     * source); 1: every row gets v_ac/cu ("uniform" run of nit/process.py:428) */
+    int variant;
 } orc_nit_cfg;
 void orc_nit_config(double p_a, double f_exc, orc_nit_cfg *cfg);
 int orc_nit_run(const orc_nit_cfg *cfg, double p_a, double f_exc, orc_tdibc *imp, int nt,
                 double *rec, double *f_out, double *rho_out);
+
+/* Acoustic-liner duct driver (the impedance-liner configuration built from the
+ * reference's components): top wall solid, bottom wall solid outside
+ * [liner_x0, liner_x1] where row 0 is a velocity_bottom_regularized boundary
+ * whose normal velocity comes from one TDIBC ODE per node (orc_tdibc_velocity,
+ * nit.c:13-47) driven by that node's lagged density; duct ends are convective
+ * (zero-gradient) columns behind a sponge in which tau ramps linearly from tau
+ * to tau_sponge over `sponge` cells.  Initial state: Gaussian density pulse
+ * rho = rho0 + amp*exp(-ln2 ((x-x0)/halfwidth)^2) at rest, populations at
+ * equilibrium in both buffers.
+ * probe_hist receives the lagged density at (probe_x, probe_y) for each step;
+ * wall_p the lagged p' at every liner node after the last step. */
+typedef struct {
+    int Nx, Ny, liner_x0, liner_x1, sponge, probe_x, probe_y;
+    double tau, tau_sponge, rho0, amp, x0, halfwidth, gain;
+    orc_tdibc imp;   /* coefficients; ODE state starts at zero */
+} orc_liner_cfg;
+void orc_liner_setup(const orc_liner_cfg *c, double *f, double *rho, double *u, double *v,
+                     double *omega, int *mask);
+void orc_liner_run(const orc_liner_cfg *c, int nt, double *probe_hist, double *wall_p,
+                   double *f_out, double *rho_out);
 
 #ifdef __cplusplus
 }
