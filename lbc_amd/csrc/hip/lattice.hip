@@ -237,6 +237,7 @@ extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double 
 {
     if (!L || !mask || !omega) return lbmx_fail(LBMX_ERR_ARG, "null argument");
     if (L->cfg.use_ibb && !phi) return lbmx_fail(LBMX_ERR_ARG, "use_ibb requires phi");
+    if (int prc = lbmx_ensure_push(L)) return prc;
     const int Nx = L->Nx, Ny = L->Ny, pitch = L->pitch;
     const size_t nn = (size_t)Nx * Ny;
     static const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
@@ -368,6 +369,7 @@ extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double 
 extern "C" int lbmx_set_uniform(lbmx_lattice *L, double omega)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (int prc = lbmx_ensure_push(L)) return prc;
     HIPCHK(hipStreamSynchronize(L->stream));
     hipFree(L->cls); L->cls = nullptr;
     hipFree(L->lut); L->lut = nullptr;
@@ -398,6 +400,7 @@ extern "C" int lbmx_set_boundaries(lbmx_lattice *L, const int *types, const int 
         int rc = lbmx_bc_check(types[b], index[b], L->Nx, L->Ny);
         if (rc) return rc;
     }
+    if (int prc = lbmx_ensure_push(L)) return prc;
     HIPCHK(hipStreamSynchronize(L->stream));
     for (double *p : L->val1) hipFree(p);
     for (double *p : L->val2) hipFree(p);
@@ -588,11 +591,15 @@ void lbmx_reset_run(lbmx_lattice *L)
     L->Fx_lag = L->cfg.Fx;
     L->Fy_lag = L->cfg.Fy;
     L->other_is_ref = 1;
+    L->pull = 0;
 }
 
 extern "C" int lbmx_set_force(lbmx_lattice *L, double Fx, double Fy)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    // in gather form the next step's collision has already used the old force
+    if (L->pull && (Fx != L->cfg.Fx || Fy != L->cfg.Fy))
+        if (int prc = lbmx_ensure_push(L)) return prc;
     L->cfg.Fx = Fx;
     L->cfg.Fy = Fy;
     return LBMX_OK;
@@ -640,8 +647,8 @@ extern "C" int lbmx_download(lbmx_lattice *L, double *f, double *f_new)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
     if (!L->has_state) return lbmx_fail(LBMX_ERR_STATE, "no state uploaded");
-    int rc = LBMX_OK;
-    if (f) rc = lbmx_get_pops(L, L->cur, f);
+    int rc = lbmx_ensure_push(L);
+    if (!rc && f) rc = lbmx_get_pops(L, L->cur, f);
     if (!rc && f_new) rc = lbmx_materialize_other(L);
     if (!rc && f_new) rc = lbmx_get_pops(L, L->cur ^ 1, f_new);
     return rc;
@@ -756,6 +763,82 @@ static int launch_step(lbmx_lattice *L)
     return LBMX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// gather form for periodic all-fluid lattices (pull_kernels.h)
+// ---------------------------------------------------------------------------
+static bool pull_eligible(const lbmx_lattice *L)
+{
+    static int disabled = -1;
+    if (disabled < 0) disabled = getenv("LBMX_NO_GATHER") ? 1 : 0;
+    return !disabled && L->geom == 0 && L->cfg.per_x && L->cfg.per_y && L->nprog == 0 &&
+           !L->tdibc.on && L->probes.n == 0 && !(L->cfg.flags & LBMX_EXACT_STATE);
+}
+
+static PullArgs pull_args(const lbmx_lattice *L)
+{
+    PullArgs a;
+    a.g = lbmx_geo(L);
+    a.s = lbmx_flags(L);
+    a.F = Fields{L->R, L->U, L->V};
+    a.tiles_per_row = (L->Nx + kPullThreads - 1) / kPullThreads;
+    return a;
+}
+
+template <typename T>
+static int enter_pull(lbmx_lattice *L)
+{
+    const PullArgs a = pull_args(L);
+    dim3 grid((L->Nx + 255) / 256, L->Ny);
+    hipLaunchKernelGGL(k_collide_inplace<T>, grid, dim3(256), 0, L->stream, (T *)L->pop[L->cur],
+                       a);
+    HIPCHK(hipGetLastError());
+    L->pull = 1;
+    L->pstar = L->cur;
+    L->lag_arrays = 0;
+    return LBMX_OK;
+}
+
+template <typename T>
+static int pull_step(lbmx_lattice *L)
+{
+    const PullArgs a = pull_args(L);
+    const unsigned blocks = (unsigned)a.tiles_per_row * (unsigned)L->Ny;
+    const T *S = (const T *)L->pop[L->pstar];
+    T *D = (T *)L->pop[L->pstar ^ 1];
+    if (a.s.force)
+        hipLaunchKernelGGL((k_pull<T, true>), dim3(blocks), dim3(kPullThreads), 0, L->stream, S,
+                           D, a);
+    else
+        hipLaunchKernelGGL((k_pull<T, false>), dim3(blocks), dim3(kPullThreads), 0, L->stream,
+                           S, D, a);
+    HIPCHK(hipGetLastError());
+    L->pstar ^= 1;
+    L->last = a.s;
+    L->Fx_lag = L->cfg.Fx;
+    L->Fy_lag = L->cfg.Fy;
+    L->steps++;
+    return LBMX_OK;
+}
+
+int lbmx_ensure_push(lbmx_lattice *L)
+{
+    if (!L->pull) return LBMX_OK;
+    const Geo g = lbmx_geo(L);
+    dim3 grid((L->Nx + 255) / 256, L->Ny);
+    if (L->esize == 8)
+        hipLaunchKernelGGL(k_gather<double>, grid, dim3(256), 0, L->stream,
+                           (const double *)L->pop[L->pstar ^ 1], (double *)L->pop[L->pstar], g);
+    else
+        hipLaunchKernelGGL(k_gather<float>, grid, dim3(256), 0, L->stream,
+                           (const float *)L->pop[L->pstar ^ 1], (float *)L->pop[L->pstar], g);
+    HIPCHK(hipGetLastError());
+    L->cur = L->pstar;
+    L->pull = 0;
+    L->other_is_ref = 1;
+    L->lag_arrays = 0;
+    return LBMX_OK;
+}
+
 extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
@@ -763,8 +846,22 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
     for (size_t b = 0; b < L->vals_set.size(); b++)
         if (lbmx_bc_nvals(L->types[b]) && !L->vals_set[b])
             return lbmx_fail(LBMX_ERR_STATE, "boundary %zu has no values", b);
+    if (nsteps <= 0) return LBMX_OK;
+    const bool f64 = (L->esize == 8);
+    if (!L->pull && nsteps >= 2 && pull_eligible(L)) {
+        int rc = f64 ? enter_pull<double>(L) : enter_pull<float>(L);
+        if (rc) return rc;
+    }
+    if (L->pull && !pull_eligible(L)) {
+        int rc = lbmx_ensure_push(L);
+        if (rc) return rc;
+    }
     for (int i = 0; i < nsteps; i++) {
-        int rc = (L->esize == 8) ? launch_step<double>(L) : launch_step<float>(L);
+        int rc;
+        if (L->pull)
+            rc = f64 ? pull_step<double>(L) : pull_step<float>(L);
+        else
+            rc = f64 ? launch_step<double>(L) : launch_step<float>(L);
         if (rc) return rc;
     }
     return LBMX_OK;
@@ -803,6 +900,7 @@ extern "C" int lbmx_set_tdibc(lbmx_lattice *L, const lbmx_tdibc *c)
         return lbmx_fail(LBMX_ERR_ARG, "tdibc: probe index %d out of range", c->probe_index);
     if (c->capacity < 0) return lbmx_fail(LBMX_ERR_ARG, "tdibc: negative capacity");
     HIPCHK(hipStreamSynchronize(L->stream));
+    if (int prc = lbmx_ensure_push(L)) return prc;
     tdibc_free(L);
     TdibcParams p{};
     p.which = c->which;
@@ -892,6 +990,7 @@ extern "C" int lbmx_set_probes(lbmx_lattice *L, const int *x, const int *y, int 
 {
     if (!L || n < 0 || capacity < 0 || (n > 0 && (!x || !y)))
         return lbmx_fail(LBMX_ERR_ARG, "bad probe list");
+    if (int prc = lbmx_ensure_push(L)) return prc;
     std::vector<long long> nodes(n);
     for (int i = 0; i < n; i++) {
         if (x[i] < 0 || x[i] >= L->Nx || y[i] < 0 || y[i] >= L->Ny)
@@ -1013,6 +1112,7 @@ extern "C" int lbmx_moments(lbmx_lattice *L, double *rho, double *u, double *v)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
     if (!L->has_state) return lbmx_fail(LBMX_ERR_STATE, "no state uploaded");
+    if (int prc = lbmx_ensure_push(L)) return prc;
     // before the first step the arrays are the uploaded ones, as the reference's
     // would be; afterwards compute_macroscopic_fields' result of the last step
     const double *r = L->R, *uu = L->U, *vv = L->V;

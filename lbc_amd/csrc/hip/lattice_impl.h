@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "lbmx.h"
+#include "pull_kernels.h"
 #include "step_kernels.h"
 #include "tdibc_kernels.h"
 
@@ -62,6 +63,9 @@ struct lbmx_lattice {
     double *Ro = nullptr, *Uo = nullptr, *Vo = nullptr;
     int other_is_ref = 0;        // the non-current buffer holds the reference's f
     lbmx::StepFlags last{};      // flags of the last step run
+    // gather form (pull_kernels.h): buffers hold post-collision populations;
+    // pop[pstar] is the one the next step streams from
+    int pull = 0, pstar = 0;
 
     // time-domain impedance boundary controller (lbmx_set_tdibc)
     struct {
@@ -97,3 +101,5 @@ int lbmx_get_field(lbmx_lattice *L, double *host, const double *dev);
 int lbmx_launch_moments(lbmx_lattice *L);
 int lbmx_materialize_other(lbmx_lattice *L);
 void lbmx_reset_run(lbmx_lattice *L);
+// Returns the lattice to the reference's two-buffer state if it runs in gather form.
+int lbmx_ensure_push(lbmx_lattice *L);

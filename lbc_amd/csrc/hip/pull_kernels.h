@@ -1,0 +1,124 @@
+// pull_kernels.h -- gather-form fast path for periodic, all-fluid lattices.
+//
+// The push step (k_bulk) reads each node's own populations and scatters the
+// post-collision values to the neighbours; every store is shifted by one node,
+// which on MI355X costs ~25% of achievable bandwidth.  For a lattice without
+// walls, boundary table or stale edges the same arithmetic can run in gather
+// form instead: the buffers then hold POST-collision populations f*, and one
+// launch per step gathers f(x) = f*(x - c_k) (streaming.c:21-42 with both axes
+// wrapping), forms the lagged moments of the gathered node, collides
+// (collision.c:24-37) and stores aligned, non-temporal f* of the next step.
+// Rounding is identical to the push path: the same moments_ref / collide_ref
+// (FP64) or moments_f32 / collide_f32 (FP32-shifted) run on the same values.
+//
+// Conversions (bit-exact):
+//   to gather form   k_collide_inplace: f*_k = collide(f_k) in the current buffer
+//   back to the reference's buffers   k_gather: f_k = gather(f*_{k-1}) into the
+//                    buffer that held f*_k; the other buffer keeps f*_{k-1},
+//                    which is exactly the reference's f_new after the swap.
+#pragma once
+#include "step_kernels.h"
+
+namespace lbmx {
+
+constexpr int kPullThreads = 256;
+
+struct PullArgs {
+    Geo g;
+    StepFlags s;
+    Fields F;
+    int tiles_per_row;
+};
+
+template <typename T>
+__device__ __forceinline__ void gather_node(const T *__restrict__ S, const Geo &g, int x, int y,
+                                            T f[Q])
+{
+    const int xm = (x == 0) ? g.Nx - 1 : x - 1, xp = (x == g.Nx - 1) ? 0 : x + 1;
+    const int ym = (y == 0) ? g.Ny - 1 : y - 1, yp = (y == g.Ny - 1) ? 0 : y + 1;
+    const size_t r0 = (size_t)y * g.pitch, rm = (size_t)ym * g.pitch, rp = (size_t)yp * g.pitch;
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
+        const int xs = cx == 1 ? xm : (cx == -1 ? xp : x);        // source x - c_x
+        const size_t row = cy == 1 ? rm : (cy == -1 ? rp : r0);   // source y - c_y
+        f[k] = S[k * g.plane + row + xs];
+    }
+}
+
+__device__ __forceinline__ void store_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+
+template <typename T, bool FORCE>
+__global__ void __launch_bounds__(kPullThreads) k_pull(const T *__restrict__ S, T *__restrict__ D,
+                                                      PullArgs a)
+{
+    const Geo &g = a.g;
+    const int y = blockIdx.x / a.tiles_per_row;
+    const int x = (blockIdx.x % a.tiles_per_row) * kPullThreads + threadIdx.x;
+    if (x >= g.Nx) return;
+    T p[Q];
+    gather_node(S, g, x, y, p);
+    if constexpr (sizeof(T) == 8) {
+        double *f = reinterpret_cast<double *>(p);
+        const Moments m = moments_ref(f, a.s.Fx_lag, a.s.Fy_lag);
+        collide_ref(f, m, g.om, FORCE, a.s.Fx, a.s.Fy);
+    } else {
+        float *q = reinterpret_cast<float *>(p);
+        const float r0 = (float)g.rho0;
+        const MomentsF m = moments_f32(q, r0, (float)a.s.Fx_lag, (float)a.s.Fy_lag);
+        collide_f32(q, m, r0, (float)g.om, FORCE, (float)a.s.Fx, (float)a.s.Fy);
+    }
+    const size_t n = (size_t)y * g.pitch + x;
+#pragma unroll
+    for (int k = 0; k < Q; k++) store_nt(&D[k * g.plane + n], p[k]);
+}
+
+// f*_k = collide(f_k) in place (lagged moments: uploaded arrays on the first
+// step after an upload, else the node's own moments).
+template <typename T>
+__global__ void __launch_bounds__(256) k_collide_inplace(T *__restrict__ S, PullArgs a)
+{
+    const Geo &g = a.g;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= g.Nx) return;
+    const size_t n = (size_t)y * g.pitch + x;
+    T p[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) p[k] = S[k * g.plane + n];
+    if constexpr (sizeof(T) == 8) {
+        double *f = reinterpret_cast<double *>(p);
+        const Moments m = a.s.lag_arrays ? Moments{a.F.R[n], a.F.U[n], a.F.V[n]}
+                                         : moments_ref(f, a.s.Fx_lag, a.s.Fy_lag);
+        collide_ref(f, m, g.om, a.s.force, a.s.Fx, a.s.Fy);
+    } else {
+        float *q = reinterpret_cast<float *>(p);
+        const float r0 = (float)g.rho0;
+        MomentsF m;
+        if (a.s.lag_arrays) {
+            m.dr = (float)(a.F.R[n] - g.rho0);
+            m.ux = (float)a.F.U[n];
+            m.uy = (float)a.F.V[n];
+        } else {
+            m = moments_f32(q, r0, (float)a.s.Fx_lag, (float)a.s.Fy_lag);
+        }
+        collide_f32(q, m, r0, (float)g.om, a.s.force, (float)a.s.Fx, (float)a.s.Fy);
+    }
+#pragma unroll
+    for (int k = 0; k < Q; k++) S[k * g.plane + n] = p[k];
+}
+
+// f_k = gather(f*_{k-1}) (streaming only).
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather(const T *__restrict__ S, T *__restrict__ D, Geo g)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= g.Nx) return;
+    T p[Q];
+    gather_node(S, g, x, y, p);
+    const size_t n = (size_t)y * g.pitch + x;
+#pragma unroll
+    for (int k = 0; k < Q; k++) D[k * g.plane + n] = p[k];
+}
+
+}  // namespace lbmx

@@ -100,10 +100,11 @@ struct MomentsF {
     float dr, ux, uy;
 };
 
+// Explicit fmaf() everywhere a fused multiply-add is wanted (and no contraction
+// pragma), so every kernel that inlines these rounds identically.
 __device__ __forceinline__ MomentsF moments_f32(const float g[Q], float rho0, float Fx,
                                                 float Fy)
 {
-#pragma clang fp contract(fast)
     float dr = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
     float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
     float jy = (g[2] - g[4]) + (g[5] + g[6]) - (g[7] + g[8]);
@@ -112,29 +113,33 @@ __device__ __forceinline__ MomentsF moments_f32(const float g[Q], float rho0, fl
     float inv = 1.0f / rho;
     MomentsF m;
     m.dr = rho - rho0;
-    m.ux = (jx + 0.5f * Fx) * inv;
-    m.uy = (jy + 0.5f * Fy) * inv;
+    m.ux = __fmaf_rn(0.5f, Fx, jx) * inv;
+    m.uy = __fmaf_rn(0.5f, Fy, jy) * inv;
     return m;
 }
 
 __device__ __forceinline__ void collide_f32(float g[Q], const MomentsF &m, float rho0, float om,
                                             bool force, float Fx, float Fy)
 {
-#pragma clang fp contract(fast)
-    float rho = rho0 + m.dr;
-    float usq15 = 1.5f * (m.ux * m.ux + m.uy * m.uy);
-    float uF3 = 3.0f * (m.ux * Fx + m.uy * Fy);
-    float gf = 1.0f - 0.5f * om;
+    const float rho = rho0 + m.dr;
+    const float usq15 = 1.5f * __fmaf_rn(m.ux, m.ux, m.uy * m.uy);
+    const float uF3 = 3.0f * __fmaf_rn(m.ux, Fx, m.uy * Fy);
+    const float gf = __fmaf_rn(-0.5f, om, 1.0f);
 #pragma unroll
     for (int k = 0; k < Q; k++) {
         const float w = (float)D2Q9::w[k];
-        const float cx = (float)D2Q9::cx[k], cy = (float)D2Q9::cy[k];
-        float cu = cx * m.ux + cy * m.uy;
-        float geq = w * (m.dr + rho * (cu * (3.0f + 4.5f * cu) - usq15));
-        float r = g[k] - om * (g[k] - geq);
+        const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
+        // c.u and c.F with c in {-1,0,1}: exact sums/differences
+        const float cu = (cx == 0 ? 0.0f : (cx > 0 ? m.ux : -m.ux)) +
+                         (cy == 0 ? 0.0f : (cy > 0 ? m.uy : -m.uy));
+        const float t = __fmaf_rn(cu, __fmaf_rn(4.5f, cu, 3.0f), -usq15);
+        const float geq = w * __fmaf_rn(rho, t, m.dr);
+        float r = __fmaf_rn(-om, g[k] - geq, g[k]);
         if (force) {
-            float cF = cx * Fx + cy * Fy;
-            r += gf * w * (3.0f * cF + 9.0f * cu * cF - uF3);
+            const float cF = (cx == 0 ? 0.0f : (cx > 0 ? Fx : -Fx)) +
+                             (cy == 0 ? 0.0f : (cy > 0 ? Fy : -Fy));
+            const float s = __fmaf_rn(9.0f * cu, cF, __fmaf_rn(3.0f, cF, -uF3));
+            r = __fmaf_rn(gf * w, s, r);
         }
         g[k] = r;
     }

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from cases import all_scenarios
+from cases import Scenario, all_scenarios
 from parity import FP64_RTOL, oracle_lattice, rel_err, same_bits
 
 pytestmark = pytest.mark.gpu
@@ -88,6 +88,56 @@ def test_fp64_bit_identical(sc):
     assert same_bits(got["rho"], ref.rho)
     assert same_bits(got["u"], ref.u)
     assert same_bits(got["v"], ref.v)
+
+
+GATHER_CASES = [
+    Scenario("tgv_gather", 40, 24, [("periodic_x", 0), ("periodic_y", 0)], 31),
+    Scenario("tgv_gather_force", 36, 28, [("periodic_x", 0), ("periodic_y", 0)], 32,
+             Fx=1e-5, Fy=-2e-5),
+    Scenario("tgv_gather_odd", 75, 19, [("periodic_y", 0), ("periodic_x", 0)], 33),
+]
+
+
+def _gather_run(sc, state, chunks, precision=0):
+    """Runs the chunks of steps with single lbmx_step calls (>= 2 steps use the
+    gather form), reading moments between chunks (which returns to the
+    reference's buffers); returns (f, f_new, rho, u, v)."""
+    from lbc_amd import native
+    lat = native.Lattice(sc.Nx, sc.Ny, precision=precision, per_x=sc.per_x, per_y=sc.per_y,
+                         Fx=sc.Fx, Fy=sc.Fy)
+    lat.set_geometry(sc.mask, sc.omega, sc.phi)
+    lat.set_boundaries(sc.bc_names)
+    lat.upload(*state)
+    for n in chunks:
+        lat.step(n)
+        rho, u, v = lat.moments()
+    f, fn = lat.download(both=True)
+    lat.close()
+    return f, fn, rho, u, v
+
+
+@pytest.mark.parametrize("sc", GATHER_CASES, ids=[s.name for s in GATHER_CASES])
+@pytest.mark.parametrize("chunks", [(9,), (3, 1, 5), (1, 2, 2, 4)], ids=["9", "3-1-5", "1-2-2-4"])
+def test_gather_form_bit_identical(sc, chunks):
+    """Periodic all-fluid lattices run multi-step calls in gather form (buffers
+    hold post-collision populations); the result, both reference buffers
+    included, must equal the oracle bit for bit."""
+    state = sc.initial_state()
+    ref, _ = run_oracle(sc, state, sum(chunks))
+    f, fn, rho, u, v = _gather_run(sc, state, chunks)
+    assert same_bits(f, ref.f)
+    assert same_bits(fn, ref.f_new)
+    assert same_bits(rho, ref.rho) and same_bits(u, ref.u) and same_bits(v, ref.v)
+
+
+@pytest.mark.parametrize("sc", GATHER_CASES, ids=[s.name for s in GATHER_CASES])
+def test_gather_form_fp32_equals_push(sc):
+    """FP32-shifted: gather form and push form round identically."""
+    state = sc.initial_state()
+    a = _gather_run(sc, state, (8,), precision=1)
+    b = _gather_run(sc, state, (1,) * 8, precision=1)
+    for x, y in zip(a, b):
+        assert same_bits(x, y)
 
 
 # FP32-shifted storage against the FP64 oracle: relative L2 error of the
