@@ -147,7 +147,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=2048)
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
+                    help="only run one plain device-to-device copy of GIB GiB (torch) and "
+                         "exit: the known-size reference for scaling rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        import torch
+        n = int(args.calibrate_copy * (1 << 30)) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": 8 * n}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is described in DESIGN.md and not built", file=sys.stderr)
