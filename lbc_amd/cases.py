@@ -69,6 +69,99 @@ def cylinder_channel(Nx=2048, Ny=512, R=32.0, xc=512.0, yc=256.0, umax=0.05, Re=
 
 
 # ---------------------------------------------------------------------------
+# Normal-incidence impedance tube with a TDIBC end wall (cases/nit/nit.c)
+# ---------------------------------------------------------------------------
+@dataclass
+class NitSpec:
+    p_a: float = 89.0          # pressure amplitude [Pa]
+    f_exc: float = 2000.0      # excitation frequency [Hz]
+    variant: int = 0           # 0: orifice rows get the area-scaled velocity (nit.c:286-300)
+                               # 1: every row gets v_ac / cu ("uniform" run)
+    n_orifices: int = 2
+    Ly: float = 0.009906
+    y1: float = 0.002528
+    y2: float = 0.007478
+    d: float = 0.00117
+    c0: float = 340.0
+    tau: float = 0.5005
+    nu: float = 1.5e-5
+    rho0: float = 1.184
+
+    def derived(self):
+        """nit.c:104-130, with the same floating-point expression order."""
+        cs = 1.0 / math.sqrt(3.0)
+        dx = 3 * self.nu * cs / (self.c0 * (self.tau - 0.5))
+        dt = dx * dx * (self.tau - 0.5) / (3 * self.nu)
+        cu = dx / dt
+        cp = 1.0 * cu * cu
+        tot_cycles = 1 + 10 + 2
+        lam = self.c0 / self.f_exc
+        Lx = tot_cycles * lam
+        kx = 2 * math.pi / lam
+        rho_a = (self.p_a / cp) / (cs * cs)
+        x_end = Lx - 1 * lam
+        x_start = x_end - 10 * lam
+        Nx = int(math.ceil(Lx / dx))
+        Ny = int(math.ceil(self.Ly / dx))
+        Nt = int(math.ceil(tot_cycles / (self.f_exc * dt)))
+        return dict(cs=cs, dx=dx, dt=dt, cu=cu, cp=cp, kx=kx, rho_a=rho_a, x_start=x_start,
+                    x_end=x_end, Nx=Nx, Ny=Ny, Nt=Nt)
+
+
+def nit_initial_fields(s: NitSpec):
+    """rho_in, U_in, V_in of nit.c:234-247 (libm sin per column, as the C code)."""
+    D = s.derived()
+    Nx, Ny, dx = D["Nx"], D["Ny"], D["dx"]
+    x = np.arange(Nx) * dx
+    env = ((x >= D["x_start"]) & (x <= D["x_end"])).astype(np.float64)
+    carrier = np.array([math.sin(D["kx"] * xi) for xi in x])
+    rho_p = D["rho_a"] * env * carrier
+    rho_in = np.broadcast_to(s.rho0 + rho_p, (Ny, Nx)).copy()
+    U_in = np.broadcast_to(D["cs"] * rho_p / s.rho0, (Ny, Nx)).copy()
+    V_in = np.zeros((Ny, Nx))
+    return rho_in, U_in, V_in
+
+
+def nit_tube(s: NitSpec, impedance, record_steps, precision=native.F64):
+    """The NIT tube on the device: Mei's initialization on the device, then a lattice
+    with periodic_y, pressure_inlet(0) and velocity_outlet_regularized(Nx-1) whose
+    velocity is driven by the lumped TDIBC controller.  impedance: dict with Zinf, A,
+    lambdas, B, C, alpha, beta (the impedance_<p>Pa.txt fit)."""
+    D = s.derived()
+    Nx, Ny = D["Nx"], D["Ny"]
+    rho_in, U_in, V_in = nit_initial_fields(s)
+    mask = np.zeros((Ny, Nx), dtype=np.int32)
+    omega = np.full((Ny, Nx), 1.0 / s.tau)
+    f = np.zeros((Ny, Nx, 9))
+    f_new = np.zeros((Ny, Nx, 9))
+    rho = np.empty((Ny, Nx))
+    u = np.empty((Ny, Nx))
+    v = np.empty((Ny, Nx))
+    native.mei_initialization(f, f_new, rho_in, U_in, V_in, rho, u, v, omega, 0, 1, 100, 1e-10,
+                              mask)
+    lat = native.Lattice(Nx, Ny, precision=precision, per_x=0, per_y=1)
+    lat.set_geometry(mask, omega)
+    lat.set_boundaries([("periodic_y", 0), ("pressure_inlet", 0),
+                        ("velocity_outlet_regularized", Nx - 1)])
+    lat.set_bc_values(1, np.full(Ny, s.rho0), None)
+    lat.set_bc_values(2, np.zeros(Ny), np.zeros(Ny))
+    y = np.arange(Ny) * D["dx"]
+    d2 = s.d / 2
+    holes = ((y >= s.y1 - d2) & (y <= s.y1 + d2)) | ((y >= s.y2 - d2) & (y <= s.y2 + d2))
+    if s.variant == 0:
+        gain = (4 * s.Ly * s.Ly) / (s.n_orifices * s.d * s.d)
+        active = holes.astype(np.int32)
+    else:
+        gain = 1.0
+        active = np.ones(Ny, dtype=np.int32)
+    lat.set_tdibc(bc=2, which=1, probe_is_column=1, probe_index=Nx - 1, per_node=0,
+                  record_node=0, capacity=record_steps, rho0=s.rho0, cs=D["cs"], cp=D["cp"],
+                  dt=D["dt"], cu=D["cu"], gain=gain, active=active, **impedance)
+    lat.upload(f, f_new, rho, u, v)
+    return lat
+
+
+# ---------------------------------------------------------------------------
 # Acoustic-liner duct with a time-domain impedance wall
 # ---------------------------------------------------------------------------
 def msd_ade(M, Rr, K):
@@ -79,12 +172,20 @@ def msd_ade(M, Rr, K):
     pair contributes -(B - iC)/(s - a) - (B + iC)/(s - conj a)."""
     alpha = -Rr / (2.0 * M)
     disc = K / M - alpha * alpha
-    if disc <= 0:
-        raise ValueError("over-damped impedance: no complex pole pair")
-    beta = math.sqrt(disc)
-    return dict(Zinf=0.0, A=(0.0, 0.0), lambdas=(-1.0, -1.0),
-                B=(-1.0 / (2.0 * M), 0.0), C=(-alpha / (2.0 * beta * M), 0.0),
-                alpha=(alpha, -1.0), beta=(beta, 0.0))
+    if disc > 1e-12 * (K / M):
+        beta = math.sqrt(disc)
+        return dict(Zinf=0.0, A=(0.0, 0.0), lambdas=(-1.0, -1.0),
+                    B=(-1.0 / (2.0 * M), 0.0), C=(-alpha / (2.0 * beta * M), 0.0),
+                    alpha=(alpha, -1.0), beta=(beta, 0.0))
+    # over-damped (or critically damped, split slightly): two real poles r1, r2;
+    # Y = a1/(s - r1) + a2/(s - r2), a_i = r_i / (M (r_i - r_j)), and a real pole
+    # enters the ODE as -A/(s - lambda), so A_i = -a_i, lambda_i = r_i
+    root = math.sqrt(max(-disc, 1e-6 * (K / M)))
+    r1, r2 = alpha + root, alpha - root
+    a1 = r1 / (M * (r1 - r2))
+    a2 = r2 / (M * (r2 - r1))
+    return dict(Zinf=0.0, A=(-a1, -a2), lambdas=(r1, r2), B=(0.0, 0.0), C=(0.0, 0.0),
+                alpha=(-1.0, -1.0), beta=(0.0, 0.0))
 
 
 @dataclass
@@ -116,7 +217,11 @@ class LinerSpec:
 def liner_arrays(s: LinerSpec):
     """(f, rho, u, v, omega, mask) of the liner duct's initial state."""
     j, i = np.mgrid[0:s.Ny, 0:s.Nx]
-    rho = s.rho0 + s.amp * np.exp(-math.log(2.0) * ((i - s.x0) / s.halfwidth) ** 2)
+    # libm exp per column (the pulse depends on x only), as a C driver computes it
+    pulse = np.array([s.rho0 + s.amp * math.exp(-math.log(2.0) * ((x - s.x0) / s.halfwidth)
+                                                * ((x - s.x0) / s.halfwidth))
+                      for x in range(s.Nx)])
+    rho = np.broadcast_to(pulse, (s.Ny, s.Nx)).copy()
     u = np.zeros((s.Ny, s.Nx))
     v = np.zeros((s.Ny, s.Nx))
     wall = (j == s.Ny - 1) | ((j == 0) & ((i < s.liner_x0) | (i > s.liner_x1)))

@@ -87,6 +87,8 @@ def _bind():
     L.lbmx_tdibc_history.argtypes = [vp, _dp, i]
     L.lbmx_set_probes.argtypes = [vp, _ip, _ip, i, i]
     L.lbmx_probe_history.argtypes = [vp, _dp, i]
+    L.lbmx_mei_initialization.argtypes = [i, i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                          i, i, i, d, _ip]
     L.lbmx_forces.argtypes = [vp, _dp, _dp]
     L.lbmx_moments.argtypes = [vp, _dp, _dp, _dp]
     L.lbmx_download.argtypes = [vp, _dp, _dp]
@@ -111,6 +113,21 @@ def check(rc):
 
 def device_name():
     return _bind().lbmx_device_name().decode()
+
+
+def mei_initialization(f, f_new, rho_in, U_in, V_in, rho, u, v, omega, per_x, per_y, iter_max,
+                       tol, mask):
+    """Mei's consistent initialization on the device (lbmx_mei_initialization);
+    arrays are updated in place as the reference function updates them.
+    Returns the number of iterations run."""
+    L = _bind()
+    Ny, Nx = rho.shape
+    n = L.lbmx_mei_initialization(Nx, Ny, _d(f), _d(f_new), _d(rho_in), _d(U_in), _d(V_in),
+                                  _d(rho), _d(u), _d(v), _d(omega), int(per_x), int(per_y),
+                                  int(iter_max), float(tol), _i(mask))
+    if n < 0:
+        check(n)
+    return n
 
 
 class Lattice:

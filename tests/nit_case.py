@@ -35,9 +35,23 @@ def golden_rows(n=None):
     return rows
 
 
+def _cfmt(x):
+    """printf("%lf") of one double, including glibc's signed "nan"/"-nan"."""
+    if x != x:
+        return "-nan" if np.signbit(x) else "nan"
+    return "%f" % x
+
+
 def format_rows(rec):
-    """nit.c:273 formatting ("%lf" = "%.6f") of (t, p', v_ac) rows."""
-    return ["%f %f %f" % (t, p, v) for t, p, v in rec]
+    """nit.c:273 formatting ("%lf %lf %lf") of (t, p', v_ac) rows."""
+    return [" ".join(_cfmt(c) for c in row) for row in rec]
+
+
+def reference_rows():
+    """The first rows printed by the reference's own nit driver (see
+    tests/golden/make_nit_ref_fixture.py)."""
+    p = os.path.join(os.path.dirname(GOLDEN), "nit_ref_2000Hz_89Pa_first64.txt")
+    return open(p).read().splitlines()
 
 
 def impedance_fields():
