@@ -5,6 +5,7 @@ Arrays follow the reference's conventions: populations ``[Ny, Nx, 9]`` float64
 """
 from __future__ import annotations
 
+import ctypes
 import ctypes as C
 
 import numpy as np
@@ -205,7 +206,7 @@ class Lattice:
         if active is not None:
             self._tdibc_active = np.ascontiguousarray(active, dtype=np.int32)
             t.active = _i(self._tdibc_active)
-        check(self._L.lbmx_set_tdibc(self.h, C.byref(t)))
+        check(self._L.lbmx_set_tdibc(self.h, ctypes.byref(t)))
         self._tdibc_capacity = capacity
 
     def tdibc_history(self):
