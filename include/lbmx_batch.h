@@ -1,0 +1,190 @@
+/*
+ * lbmx_batch.h -- many independent lattices advanced together on one GPU, and a
+ * request server built on them.
+ *
+ * The reference runs one lattice per process (cases/<case>/<case>.c main loop,
+ * one main_lbm call per step, src/core/main_lbm.c:5-62).  This header adds what
+ * that design cannot express and has no counterpart in the reference: a batch
+ * of lattices of uneven sizes whose time steps run as ONE persistent launch per
+ * step (captured in a HIP graph), and a server that admits solve requests into
+ * such a batch while it runs (continuous batching).
+ *
+ * Each lattice is described by an lbmx_case: a program made of the reference's
+ * own building blocks, applied in the reference's order within a step
+ * (main_lbm.c:31-61):
+ *   collision      BGK + Guo body force (collision.c:24-37), lagged moments
+ *   walls          solid rows/columns with half-way bounce-back (bounce_back.c)
+ *   streaming      pull with periodic wrap (streaming.c:21-42)
+ *   boundary table, in this order when present:
+ *     convective_inlet(0), convective_outlet(Nx-1)          (convective_*.c)
+ *     velocity_bottom_regularized(0) over the bottom-wall opening, its normal
+ *       velocity from a per-node time-domain impedance ODE   (cases/nit/nit.c:13-47)
+ *     velocity_top(Ny-1)                                     (velocity_top.c)
+ *   moments        compute_macroscopic_fields.c:19-48
+ * A case run in a batch gives, in LBMX_F64, bit for bit the result of the same
+ * program run by the reference (and by lbmx_lattice); in LBMX_F32_SHIFTED, bit
+ * for bit the result of an LBMX_F32_SHIFTED lbmx_lattice.  Results never depend
+ * on the other lattices in the batch, on the launch configuration or on the
+ * HIP-graph chunking.
+ *
+ * Same error convention as lbmx.h (LBMX_OK or a negative code, lbmx_last_error).
+ */
+#ifndef LBMX_BATCH_H
+#define LBMX_BATCH_H
+
+#include <stddef.h>
+
+#include "lbmx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* lbmx_case.walls: solid rows / columns (the mask of the reference drivers). */
+#define LBMX_WALL_BOTTOM 1u   /* row 0 (except the opening gap_x0..gap_x1) */
+#define LBMX_WALL_TOP 2u      /* row Ny-1 */
+#define LBMX_WALL_LEFT 4u     /* column 0 */
+#define LBMX_WALL_RIGHT 8u    /* column Nx-1 */
+
+/* lbmx_case.init: the state the first step starts from (both buffers hold the
+ * equilibrium of these moments; the first collision reads them as its lagged
+ * moment arrays, collision.c:24-32). */
+#define LBMX_INIT_PULSE 0     /* rho = rho0 + amp*exp(-ln2*((x-x0)/halfwidth)^2), u = v = 0 */
+#define LBMX_INIT_TGV 1       /* Taylor-Green vortex of lbmx_init_taylor_green with
+                                 kx = 2pi/Nx, ky = 2pi/Ny */
+#define LBMX_INIT_FIELDS 2    /* rho/u/v arrays given to lbmx_batch_set_fields */
+
+/* Multipole admittance of the impedance boundary (cases/nit/nit.c:13-47,
+ * impedance_*Pa.txt:1-7): two real poles (A, lambdas), two complex pairs
+ * (B, C, alpha, beta), and Zinf. */
+typedef struct {
+    double Zinf, A[2], lambdas[2], B[2], C[2], alpha[2], beta[2];
+} lbmx_impedance;
+
+typedef struct {
+    int Nx, Ny;                /* 3 <= Ny, 3 <= Nx, Nx*Ny*9 < 2^31 */
+    int per_x, per_y;          /* periodic axes */
+    unsigned walls;            /* LBMX_WALL_* */
+    int gap_x0, gap_x1;        /* opening of the bottom wall (inclusive); none if gap_x1 < gap_x0 */
+    double tau;                /* relaxation time */
+    int sponge;                /* >0: tau ramps linearly to tau_sponge over `sponge`
+                                  columns at both ends (the liner duct's absorbing layers) */
+    double tau_sponge;
+    double Fx, Fy;             /* Guo body force */
+    int top_velocity;          /* velocity_top(Ny-1) with val1 = U_top, val2 = V_top */
+    double U_top, V_top;
+    int convective_ends;       /* convective_inlet(0) and convective_outlet(Nx-1) */
+    int liner;                 /* impedance wall on the opening: velocity_bottom_regularized
+                                  with val1 = 0 and val2 = gain * v(p'), v from the ODE of
+                                  `imp` driven by p' = cs^2 (rho_lagged - rho0) per node */
+    double gain;
+    lbmx_impedance imp;
+    int init;                  /* LBMX_INIT_* */
+    double rho0;               /* reference density (and FP32 shift) */
+    double amp, x0, halfwidth; /* LBMX_INIT_PULSE */
+    double u0;                 /* LBMX_INIT_TGV */
+    int nprobe;                /* <= 4 density probes: lagged rho before every step */
+    int probe_x[4], probe_y[4];
+    int steps;                 /* length of the solve */
+    int keep_moments;          /* store rho/u/v when the solve completes */
+} lbmx_case;
+
+/* Launch configuration of the persistent step kernel (chosen by lbmx_batch_tune). */
+typedef struct {
+    int threads;               /* threads per workgroup: 64, 128 or 256 */
+    int tile;                  /* nodes per tile: threads * {1, 2, 4, 8} */
+    int groups_per_cu;         /* persistent workgroups per compute unit */
+    int banded;                /* 1: contiguous tile bands per XCD; 0: round-robin */
+} lbmx_tuning;
+
+typedef struct lbmx_batch lbmx_batch;
+
+/* precision: LBMX_F64 or LBMX_F32_SHIFTED.  budget: bytes of device memory the
+ * batch may hold for its lattices (one arena, allocated here). */
+LBMX_API int lbmx_batch_create(int precision, size_t budget, lbmx_batch **out);
+LBMX_API void lbmx_batch_destroy(lbmx_batch *b);
+
+/* Device bytes one case occupies in a batch of this precision. */
+LBMX_API size_t lbmx_case_bytes(int precision, const lbmx_case *c);
+/* Checks a case against the rules lbmx_batch_add enforces (no device calls). */
+LBMX_API int lbmx_case_check(const lbmx_case *c);
+
+/* Adds a lattice; it starts stepping with the next lbmx_batch_run.
+ * LBMX_ERR_NOMEM when the budget cannot hold it (nothing is changed then). */
+LBMX_API int lbmx_batch_add(lbmx_batch *b, const lbmx_case *c, int *id);
+/* Initial moments [Ny][Nx] for an LBMX_INIT_FIELDS case, before its first run. */
+LBMX_API int lbmx_batch_set_fields(lbmx_batch *b, int id, const double *rho, const double *u,
+                                   const double *v);
+/* Releases a lattice and its memory. */
+LBMX_API int lbmx_batch_remove(lbmx_batch *b, int id);
+
+/* Advances every lattice nsteps steps (a lattice stops at its own `steps`).
+ * Asynchronous; whole chunks run as one HIP-graph launch. */
+LBMX_API int lbmx_batch_run(lbmx_batch *b, int nsteps);
+/* Same, timed with HIP events on the batch's stream (device milliseconds). */
+LBMX_API int lbmx_batch_time(lbmx_batch *b, int nsteps, double *ms);
+LBMX_API int lbmx_batch_synchronize(lbmx_batch *b);
+
+/* Steps completed by lattice id (as enqueued; synchronize before reading results). */
+LBMX_API int lbmx_batch_steps_done(lbmx_batch *b, int id, int *steps);
+/* rho/u/v [Ny][Nx] at completion (keep_moments); NaN on solid nodes. */
+LBMX_API int lbmx_batch_moments(lbmx_batch *b, int id, double *rho, double *u, double *v);
+/* Probe history [(steps+1)][nprobe]: row t = lagged rho before step t. Returns rows. */
+LBMX_API int lbmx_batch_probes(lbmx_batch *b, int id, double *hist, int capacity);
+
+/* Times candidate launch configurations on the batch as it stands (every lattice
+ * must still be waiting for its first step: trial steps run on scratch state and
+ * the lattices are initialised afterwards) and keeps the fastest. */
+LBMX_API int lbmx_batch_tune(lbmx_batch *b, int trial_steps, lbmx_tuning *chosen);
+LBMX_API int lbmx_batch_set_tuning(lbmx_batch *b, const lbmx_tuning *t);
+LBMX_API int lbmx_batch_get_tuning(lbmx_batch *b, lbmx_tuning *t);
+/* Steps per HIP-graph launch (default 32). */
+LBMX_API int lbmx_batch_set_chunk(lbmx_batch *b, int steps);
+
+/* ---- server: continuous batching of solve requests ------------------------------
+ * A worker thread owns a batch.  Submitted requests wait in arrival order and are
+ * admitted whenever the memory budget can hold them; between graph chunks the
+ * finished lattices leave and the waiting ones join, so requests that arrive
+ * while others run overlap with them on the device.  A finished request's probe
+ * histories are reduced on the device to spectra (lbmx_tl_spec) and its memory
+ * is released. */
+typedef struct {
+    int nfreq;                 /* <= 64 frequencies, linear from f_lo to f_hi (cycles/step) */
+    double f_lo, f_hi;
+    int up_probe, down_probe;  /* probe indices of the case */
+    int up_window;             /* samples of the upstream probe used (incident pulse) */
+} lbmx_tl_spec;
+
+typedef struct {
+    long long tag;             /* as submitted */
+    int status;                /* LBMX_OK or an error code */
+    int steps;
+    double t_submit, t_admit, t_done;   /* seconds on the server's monotonic clock */
+    int nfreq;
+    double tl_db[64];          /* 20 log10(|P_up(f)| / |P_down(f)|) */
+    double up_abs[64], down_abs[64];
+} lbmx_result;
+
+typedef struct lbmx_server lbmx_server;
+
+/* precision/budget as for a batch; chunk = steps per graph launch. */
+LBMX_API int lbmx_server_create(int precision, size_t budget, int chunk,
+                                const lbmx_tuning *tuning, lbmx_server **out);
+/* Queues one request (thread-safe, returns at once). */
+LBMX_API int lbmx_server_submit(lbmx_server *s, const lbmx_case *c, const lbmx_tl_spec *tl,
+                                long long tag);
+/* Copies up to max finished results (oldest first); returns the count. */
+LBMX_API int lbmx_server_poll(lbmx_server *s, lbmx_result *out, int max);
+/* Seconds on the server's clock (the clock of t_submit / t_done). */
+LBMX_API double lbmx_server_clock(lbmx_server *s);
+/* Counters: requests waiting, running, finished; peak bytes held. */
+LBMX_API int lbmx_server_stats(lbmx_server *s, int *waiting, int *running, int *finished,
+                               size_t *peak_bytes);
+/* Stops the worker after the running and waiting requests finish, then frees. */
+LBMX_API void lbmx_server_destroy(lbmx_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

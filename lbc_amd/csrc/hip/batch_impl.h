@@ -1,0 +1,75 @@
+// batch_impl.h -- internal definition of lbmx_batch (shared by batch.hip and server.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "batch_kernels.h"
+#include "lattice_impl.h"
+#include "lbmx_batch.h"
+
+constexpr int kBatchMaxSlots = 8192;
+
+// First-fit device arena with coalescing free list; one block per lattice.
+struct BatchArena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0, peak = 0;
+    std::map<size_t, size_t> free_;   // offset -> bytes
+    int init(size_t bytes);
+    void *alloc(size_t bytes);
+    void release(void *p, size_t bytes);
+    void destroy();
+};
+
+// Byte offsets of one lattice's arrays inside its arena block.
+struct BatchLayout {
+    size_t buf[2], om, out[3], saved[2], state, probe, init_a, init_b, total;
+    unsigned plane;
+    int gap;
+};
+
+struct BatchStaging {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+
+struct BatchSlot {
+    int used = 0;
+    int pending = 0;          // waiting for its first step (descriptor not on the device yet)
+    int fields_set = 0;
+    lbmx_case c{};
+    BatchLayout lay{};
+    char *block = nullptr;
+    lbmx::BatchLat d{};
+    lbmx::BatchInit init{};
+};
+
+struct lbmx_batch {
+    int precision = 0;
+    size_t esize = 8;
+    hipStream_t stream = nullptr;
+    int ncu = 256;
+    BatchArena arena;
+    lbmx::BatchLat *lats_dev = nullptr;
+    lbmx::Tile *tiles_dev = nullptr;
+    long long tiles_cap = 0;
+    lbmx::BatchCtl *ctl_dev = nullptr;
+    std::vector<BatchSlot> slots;
+    std::vector<int> order;          // lattices in tile-table order
+    std::vector<lbmx::Tile> tile_host;
+    long long epoch = 0;             // host mirror of ctl->epoch (steps enqueued)
+    bool dirty = true;
+    lbmx_tuning tun{256, 512, 8, 1};
+    int chunk = 32;
+    bool use_graphs = true;
+    std::map<int, hipGraphExec_t> graphs;
+    std::vector<BatchStaging> ring;
+    int ring_next = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+int lbmx_batch_enqueue(lbmx_batch *b, int nsteps);
+const BatchSlot *lbmx_batch_slot(const lbmx_batch *b, int id);

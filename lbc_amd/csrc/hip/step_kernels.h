@@ -508,6 +508,49 @@ __device__ void bc_zou_he_column(const T *S, T *D, const BcArgs &a, const BcDesc
     }
 }
 
+// ---- per-node boundary closures shared with the batched kernels ---------------
+// velocity_top.c:28-39 on one node's post-streaming populations: slots 4, 7, 8.
+__device__ __forceinline__ void zou_he_top_node(double f[Q], double U, double V)
+{
+    const double r = (f[0] + f[1] + f[3] + 2 * (f[2] + f[5] + f[6])) / (1 + V);
+    const double f4 = f[2] - 2.0 / 3.0 * r * V;
+    const double f7 = f[5] + (f[1] - f[3]) / 2.0 - 0.5 * r * U - 1.0 / 6.0 * r * V;
+    const double f8 = f[6] - (f[1] - f[3]) / 2.0 + 0.5 * r * U - 1.0 / 6.0 * r * V;
+    f[4] = f4;
+    f[7] = f7;
+    f[8] = f8;
+}
+
+// *_regularized.c:18-71 for one boundary node: fi are the post-streaming
+// populations of the neighbouring interior node, (ri, ui, vi) its lagged moments,
+// (rb, Ub, Vb) the boundary state; out receives feq(rb, Ub, Vb) + 1.5 w Q:Pi.
+__device__ __forceinline__ void regularized_node(const double fi[Q], double ri, double ui,
+                                                 double vi, double rb, double Ub, double Vb,
+                                                 double out[Q])
+{
+    const double usq_i = ui * ui + vi * vi;
+    const double usq_b = Ub * Ub + Vb * Vb;
+    double Pxx = 0.0, Pxy = 0.0, Pyy = 0.0;
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
+        double neq = fi[k] - feq_ref(D2Q9::w[k], (double)cx, (double)cy, ri, ui, vi, usq_i);
+        Pxx += (double)(cx * cx) * neq;
+        Pxy += (double)(cx * cy) * neq;
+        Pyy += (double)(cy * cy) * neq;
+    }
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
+        double eq = feq_ref(D2Q9::w[k], (double)cx, (double)cy, rb, Ub, Vb, usq_b);
+        double Qxx = (double)(cx * cx) - 1.0 / 3.0;
+        double Qxy = (double)(cx * cy);
+        double Qyy = (double)(cy * cy) - 1.0 / 3.0;
+        double neq = (3.0 / 2.0) * D2Q9::w[k] * (Qxx * Pxx + 2.0 * Qxy * Pxy + Qyy * Pyy);
+        out[k] = eq + neq;
+    }
+}
+
 template <typename T>
 __device__ void bc_zou_he_row(T *D, const BcArgs &a, const BcDesc &d)
 {
@@ -525,12 +568,10 @@ __device__ void bc_zou_he_row(T *D, const BcArgs &a, const BcDesc &d)
 #pragma unroll
         for (int k = 0; k < Q; k++) f[k] = pop_get(D, k * P + n, k, g.rho0);
         if (top) {
-            double r = (f[0] + f[1] + f[3] + 2 * (f[2] + f[5] + f[6])) / (1 + V);
-            pop_put(D, 4 * P + n, 4, f[2] - 2.0 / 3.0 * r * V, g.rho0);
-            pop_put(D, 7 * P + n, 7,
-                    f[5] + (f[1] - f[3]) / 2.0 - 0.5 * r * U - 1.0 / 6.0 * r * V, g.rho0);
-            pop_put(D, 8 * P + n, 8,
-                    f[6] - (f[1] - f[3]) / 2.0 + 0.5 * r * U - 1.0 / 6.0 * r * V, g.rho0);
+            zou_he_top_node(f, U, V);
+            pop_put(D, 4 * P + n, 4, f[4], g.rho0);
+            pop_put(D, 7 * P + n, 7, f[7], g.rho0);
+            pop_put(D, 8 * P + n, 8, f[8], g.rho0);
         } else {
             double r = (f[0] + f[1] + f[3] + 2 * (f[4] + f[7] + f[8])) / (1 - V);
             pop_put(D, 2 * P + n, 2, f[4] + 2.0 / 3.0 * r * V, g.rho0);
@@ -561,7 +602,6 @@ __device__ void bc_regularized(const T *S, T *D, const BcArgs &a, const BcDesc &
         if (solid_at(g, nb)) continue;
         const Moments mi = lagged_at(S, g, a.F, a.s, ni);
         const double ri = mi.rho, ui = mi.ux, vi = mi.uy;
-        const double usq_i = ui * ui + vi * vi;
         double rb, Ub, Vb;
         if (pressure) {
             rb = d.v1[e];
@@ -572,27 +612,12 @@ __device__ void bc_regularized(const T *S, T *D, const BcArgs &a, const BcDesc &
             Ub = d.v1[e];
             Vb = d.v2[e];
         }
-        const double usq_b = Ub * Ub + Vb * Vb;
-        double Pxx = 0.0, Pxy = 0.0, Pyy = 0.0;
+        double fi[Q], out[Q];
 #pragma unroll
-        for (int k = 0; k < Q; k++) {
-            const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
-            double neq = pop_get(D, k * P + ni, k, g.rho0) -
-                         feq_ref(D2Q9::w[k], (double)cx, (double)cy, ri, ui, vi, usq_i);
-            Pxx += (double)(cx * cx) * neq;
-            Pxy += (double)(cx * cy) * neq;
-            Pyy += (double)(cy * cy) * neq;
-        }
+        for (int k = 0; k < Q; k++) fi[k] = pop_get(D, k * P + ni, k, g.rho0);
+        regularized_node(fi, ri, ui, vi, rb, Ub, Vb, out);
 #pragma unroll
-        for (int k = 0; k < Q; k++) {
-            const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
-            double eq = feq_ref(D2Q9::w[k], (double)cx, (double)cy, rb, Ub, Vb, usq_b);
-            double Qxx = (double)(cx * cx) - 1.0 / 3.0;
-            double Qxy = (double)(cx * cy);
-            double Qyy = (double)(cy * cy) - 1.0 / 3.0;
-            double neq = (3.0 / 2.0) * D2Q9::w[k] * (Qxx * Pxx + 2.0 * Qxy * Pxy + Qyy * Pyy);
-            pop_put(D, k * P + nb, k, eq + neq, g.rho0);
-        }
+        for (int k = 0; k < Q; k++) pop_put(D, k * P + nb, k, out[k], g.rho0);
     }
 }
 
