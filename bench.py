@@ -88,6 +88,74 @@ def cylinder_lattice(precision):
     return lat
 
 
+def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True):
+    """The batched workload: n independent lattices (Poiseuille / Couette / cavity /
+    Taylor-Green, sizes uniform in {64..1024} x {64..512}) advanced `steps` steps in
+    FP32-shifted storage, one persistent HIP-graph-captured launch per step.
+    Baseline: the same lattices one after another on the single-lattice FP64 path
+    (its plain per-step launches), timed on a seeded random sample of `sample`
+    lattices and scaled by n/sample (an unbiased estimate of the full sequential
+    time); the sample also gives the FP32-vs-FP64 velocity error."""
+    from lbc_amd import batch as B
+    from lbc_amd import native
+    cases = B.batched_workload(n, steps=steps, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    pick = sorted(rng.choice(n, size=min(sample, n), replace=False).tolist())
+    for i, c in enumerate(cases):
+        c.keep_moments = i in pick
+    budget = sum(B.case_bytes(c, native.F32_SHIFTED) for c in cases) + (256 << 20)
+    b = B.Batch(native.F32_SHIFTED, budget=budget, chunk=32)
+    ids = [b.add(c) for c in cases]
+    t0 = time.perf_counter()
+    tuning = b.tune(trial_steps=3) if tune else b.tuning()
+    t_tune = time.perf_counter() - t0
+    b.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = b.time(steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    cells = sum(c.Nx * c.Ny for c in cases)
+    gbs = 72.0 * cells * steps / (dev_ms * 1e-3) / 1e9
+    log(f"batched {n} lattices x {steps} steps FP32: {wall:.2f} s wall, {dev_ms / 1e3:.2f} s "
+        f"device, {n / wall:.1f} solves/s, {cells * steps / wall / 1e6:.0f} MLUPS, "
+        f"{gbs:.0f} GB/s, tuning {tuning} ({t_tune:.1f} s)")
+
+    seq = 0.0
+    errs = []
+    for i in pick:
+        c = cases[i]
+        lat = c.single_lattice(native.F64)
+        lat.synchronize()
+        t1 = time.perf_counter()
+        lat.step(c.steps)
+        lat.synchronize()
+        seq += time.perf_counter() - t1
+        r64, u64, v64 = lat.moments()
+        lat.close()
+        r32, u32, v32 = b.moments(ids[i])
+        fl = ~np.isnan(r64)
+        num = ((u32 - u64)[fl] ** 2 + (v32 - v64)[fl] ** 2).sum()
+        den = (u64[fl] ** 2 + v64[fl] ** 2).sum()
+        errs.append(math.sqrt(num / den) if den > 0 else 0.0)
+    b.close()
+    seq_total = seq * n / len(pick)
+    log(f"sequential FP64 (sample {len(pick)}): {seq:.2f} s -> {seq_total:.1f} s for {n}, "
+        f"{n / seq_total:.2f} solves/s; L2 velocity error max {max(errs):.2e}")
+    return {
+        "workload": f"batched_{n}_lattices_{steps}_steps_fp32_shifted",
+        "solves_per_s": round(n / wall, 2), "wall_s": round(wall, 3),
+        "device_s": round(dev_ms / 1e3, 3), "MLUPS": round(cells * steps / wall / 1e6, 1),
+        "GB_per_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+        "cells": cells, "tuning": {"threads": tuning[0], "tile": tuning[1],
+                                   "groups_per_cu": tuning[2], "order": tuning[3]},
+        "sequential_fp64_solves_per_s": round(n / seq_total, 3),
+        "sequential_sample": len(pick),
+        "speedup_vs_sequential": round((n / wall) / (n / seq_total), 2),
+        "l2_velocity_err_vs_fp64_max": float(f"{max(errs):.3e}"),
+        "l2_velocity_err_vs_fp64_median": float(f"{float(np.median(errs)):.3e}"),
+    }
+
+
 def load_traffic(tag):
     """HBM bytes per launch of the fused kernel from the committed pmc summary."""
     p = os.path.join(ROOT, "profiles", f"pmc_{tag}.json")
@@ -147,6 +215,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=2048)
+    ap.add_argument("--batched", type=int, default=1024, metavar="N",
+                    help="lattices of the batched workload (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=5000)
+    ap.add_argument("--batched-sample", type=int, default=48,
+                    help="lattices timed one by one for the sequential FP64 baseline")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run one plain device-to-device copy of GIB GiB (torch) and "
                          "exit: the known-size reference for scaling rocprofv3 "
@@ -200,6 +273,9 @@ def main():
             "MLUPS": round(2048 * 512 * 500 / wc / 1e6, 1),
             "ms_per_step": round(wc / 500 * 1e3, 4)}
         log(f"cylinder 2048x512 FP64: {secondary['cylinder_fp64']}")
+    if args.batched > 0:
+        secondary["batched"] = batched_bench(args.batched, args.batched_steps,
+                                             args.batched_sample)
 
     cpu = None
     if not args.no_cpu:

@@ -89,12 +89,14 @@ typedef struct {
     int keep_moments;          /* store rho/u/v when the solve completes */
 } lbmx_case;
 
-/* Launch configuration of the persistent step kernel (chosen by lbmx_batch_tune). */
+/* Launch configuration of the persistent interior kernel (chosen by lbmx_batch_tune). */
 typedef struct {
     int threads;               /* threads per workgroup: 64, 128 or 256 */
-    int tile;                  /* nodes per tile: threads * {1, 2, 4, 8} */
+    int tile;                  /* nodes per tile: threads * {1 .. 16}, >= 256 */
     int groups_per_cu;         /* persistent workgroups per compute unit */
-    int banded;                /* 1: contiguous tile bands per XCD; 0: round-robin */
+    int order;                 /* tiles to workgroups: 0 interleaved (tile t on workgroup
+                                  t mod G), 1 one contiguous band per XCD, 2 one contiguous
+                                  range per workgroup */
 } lbmx_tuning;
 
 typedef struct lbmx_batch lbmx_batch;

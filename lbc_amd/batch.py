@@ -46,10 +46,10 @@ class CaseStruct(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("threads", C.c_int), ("tile", C.c_int), ("groups_per_cu", C.c_int),
-                ("banded", C.c_int)]
+                ("order", C.c_int)]
 
     def as_tuple(self):
-        return (self.threads, self.tile, self.groups_per_cu, self.banded)
+        return (self.threads, self.tile, self.groups_per_cu, self.order)
 
 
 class TlSpec(C.Structure):
@@ -404,8 +404,8 @@ class Batch:
         check(self._L.lbmx_batch_tune(self.h, int(trial_steps), C.byref(t)))
         return t.as_tuple()
 
-    def set_tuning(self, threads, tile, groups_per_cu, banded):
-        t = Tuning(threads, tile, groups_per_cu, banded)
+    def set_tuning(self, threads, tile, groups_per_cu, order):
+        t = Tuning(threads, tile, groups_per_cu, order)
         check(self._L.lbmx_batch_set_tuning(self.h, C.byref(t)))
 
     def tuning(self):

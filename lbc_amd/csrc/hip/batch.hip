@@ -247,6 +247,7 @@ static void fill_desc(lbmx_batch *b, BatchSlot &S)
     d.steps = c->steps;
     d.nprobe = c->nprobe;
     for (int i = 0; i < c->nprobe; i++) d.probe_n[i] = c->probe_y[i] * c->Nx + c->probe_x[i];
+    d.inv_nx = 1.0 / c->Nx;
     d.rho0 = c->rho0;
     d.Fx = c->Fx;
     d.Fy = c->Fy;
@@ -333,12 +334,18 @@ static int grid_of(const lbmx_batch *b)
 static int launch_step(lbmx_batch *b, int local)
 {
     const dim3 grid(grid_of(b)), block(b->tun.threads);
-    if (b->precision == LBMX_F64)
-        hipLaunchKernelGGL(k_batch_step<double>, grid, block, 0, b->stream, b->lats_dev,
-                           b->tiles_dev, b->ctl_dev, local, b->tun.tile, b->tun.banded);
-    else
-        hipLaunchKernelGGL(k_batch_step<float>, grid, block, 0, b->stream, b->lats_dev,
-                           b->tiles_dev, b->ctl_dev, local, b->tun.tile, b->tun.banded);
+    const dim3 fgrid(2 * b->ncu), fblock(kFrameThreads);
+    if (b->precision == LBMX_F64) {
+        hipLaunchKernelGGL(k_batch_interior<double>, grid, block, 0, b->stream, b->lats_dev,
+                           b->tiles_dev, b->ctl_dev, local, b->tun.tile, b->tun.order);
+        hipLaunchKernelGGL(k_batch_frame<double>, fgrid, fblock, 0, b->stream, b->lats_dev,
+                           b->tiles_dev, b->ctl_dev, local);
+    } else {
+        hipLaunchKernelGGL(k_batch_interior<float>, grid, block, 0, b->stream, b->lats_dev,
+                           b->tiles_dev, b->ctl_dev, local, b->tun.tile, b->tun.order);
+        hipLaunchKernelGGL(k_batch_frame<float>, fgrid, fblock, 0, b->stream, b->lats_dev,
+                           b->tiles_dev, b->ctl_dev, local);
+    }
     HIPCHK(hipGetLastError());
     return LBMX_OK;
 }
@@ -387,13 +394,19 @@ static int commit(lbmx_batch *b)
     const int tile = b->tun.tile;
     for (int slot : b->order) {
         const BatchSlot &S = b->slots[slot];
-        const int N = S.c.Nx * S.c.Ny;
-        for (int s0 = 0; s0 < N; s0 += tile) T.push_back(Tile{slot, s0});
+        const int ni = interior_nodes(S.c.Nx, S.c.Ny);
+        for (int s0 = 0; s0 < ni; s0 += tile) T.push_back(Tile{slot, s0});
+    }
+    const int n_interior = (int)T.size();
+    for (int slot : b->order) {
+        const BatchSlot &S = b->slots[slot];
+        const int nf = frame_nodes(S.c.Nx, S.c.Ny);
+        for (int f0 = 0; f0 < nf; f0 += kFrameTile) T.push_back(Tile{slot, f0});
     }
     if ((long long)T.size() > b->tiles_cap)
         return lbmx_fail(LBMX_ERR_NOMEM, "tile table overflow (%zu > %lld)", T.size(), b->tiles_cap);
     if (int rc = upload(b, b->tiles_dev, T.data(), T.size() * sizeof(Tile))) return rc;
-    BatchCtl ctl{b->epoch, (int)T.size(), 0};
+    BatchCtl ctl{b->epoch, n_interior, (int)T.size() - n_interior};
     if (int rc = upload(b, b->ctl_dev, &ctl, sizeof ctl)) return rc;
     b->dirty = false;
     return LBMX_OK;
@@ -486,8 +499,9 @@ LBMX_API int lbmx_batch_create(int precision, size_t budget, lbmx_batch **out)
         delete b;
         return lbmx_fail(LBMX_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
-    // every tile but each lattice's last holds >= 256 nodes of >= 18*esize bytes
-    b->tiles_cap = (long long)(budget / (18 * b->esize * 256)) + kBatchMaxSlots + 1024;
+    // every tile but the last interior and the last frame tile of a lattice holds
+    // >= 256 nodes of >= 18*esize bytes
+    b->tiles_cap = (long long)(budget / (18 * b->esize * 256)) + 2 * kBatchMaxSlots + 1024;
     rc = b->arena.init(budget);
     if (!rc && hipMalloc((void **)&b->lats_dev, sizeof(BatchLat) * kBatchMaxSlots) != hipSuccess)
         rc = lbmx_fail(LBMX_ERR_NOMEM, "descriptor table");
@@ -680,8 +694,8 @@ LBMX_API int lbmx_batch_set_tuning(lbmx_batch *b, const lbmx_tuning *t)
         t->groups_per_cu > 32)
         return lbmx_fail(LBMX_ERR_ARG, "invalid tuning (threads %d, tile %d, groups/CU %d)",
                          t->threads, t->tile, t->groups_per_cu);
+    if (t->order < 0 || t->order > 2) return lbmx_fail(LBMX_ERR_ARG, "tile order must be 0, 1 or 2");
     b->tun = *t;
-    b->tun.banded = t->banded ? 1 : 0;
     drop_graphs(b);
     b->dirty = true;
     return LBMX_OK;
@@ -715,13 +729,13 @@ LBMX_API int lbmx_batch_tune(lbmx_batch *b, int trial_steps, lbmx_tuning *chosen
     const bool graphs0 = b->use_graphs;
     b->use_graphs = false;
     std::vector<lbmx_tuning> cand;
-    for (int threads : {64, 128, 256})
+    for (int threads : {128, 256})
         for (int ilp : {1, 2, 4, 8})
             for (int g : {2, 4, 8, 16})
-                for (int banded : {0, 1}) {
+                for (int order : {0, 1, 2}) {
                     const int tile = threads * ilp;
                     if (tile < 256) continue;
-                    cand.push_back(lbmx_tuning{threads, tile, g, banded});
+                    cand.push_back(lbmx_tuning{threads, tile, g, order});
                 }
     double best = 1e300;
     lbmx_tuning pick = keep;

@@ -62,7 +62,7 @@ struct lbmx_batch {
     std::vector<lbmx::Tile> tile_host;
     long long epoch = 0;             // host mirror of ctl->epoch (steps enqueued)
     bool dirty = true;
-    lbmx_tuning tun{256, 512, 8, 1};
+    lbmx_tuning tun{256, 512, 8, 0};
     int chunk = 32;
     bool use_graphs = true;
     std::map<int, hipGraphExec_t> graphs;

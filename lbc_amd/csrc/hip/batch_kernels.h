@@ -59,19 +59,20 @@ struct BatchLat {
     int steps;
     int nprobe;
     int probe_n[4];
+    double inv_nx;           // 1 / Nx: interior index -> row
     double rho0, Fx, Fy, U_top, V_top;
     TdibcParams tp;
 };
 
 struct BatchCtl {
     long long epoch;         // steps launched so far (k_batch_advance)
-    int ntiles;
-    int pad;
+    int n_interior;          // interior tiles (tile table [0, n_interior))
+    int n_frame;             // frame tiles (tile table [n_interior, n_interior + n_frame))
 };
 
 struct Tile {
     int lat;                 // descriptor slot
-    int start;               // first node
+    int start;               // first node of the tile in its kind's numbering
 };
 
 // Initial state sources of k_batch_init.
@@ -85,6 +86,23 @@ struct BatchInit {
 };
 
 constexpr int kBatchThreads = 256;
+
+// Pointers read from a descriptor are generic to the compiler; casting them to the
+// global address space makes every access a global_load/global_store (flat ones
+// would also count on the scalar-load counter, so each descriptor read would wait
+// for all populations in flight).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> gbl(T *p)
+{
+    return (gptr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ gptr<const T> gbl(const T *p)
+{
+    return (gptr<const T>)p;
+}
 
 __device__ __forceinline__ bool b_solid(const BatchLat &L, int x, int y)
 {
@@ -114,8 +132,8 @@ __device__ __forceinline__ T b_enc(double v, int k, double rho0)
 // slots of D, which only this node's thread writes (callers gathering for another
 // node never meet a stale entry; the host validates that).
 template <typename T>
-__device__ __forceinline__ void b_gather(const BatchLat &L, const T *__restrict__ S,
-                                         const T *D, int x, int y, T f[Q])
+__device__ __forceinline__ void b_gather(const BatchLat &L, gptr<const T> __restrict__ S,
+                                         gptr<const T> D, int x, int y, T f[Q])
 {
     const int Nx = L.Nx, Ny = L.Ny;
     const unsigned P = L.plane;
@@ -152,27 +170,132 @@ __device__ __forceinline__ void b_gather(const BatchLat &L, const T *__restrict_
     }
 }
 
-__device__ __forceinline__ void b_store(double *p, double v) { __builtin_nontemporal_store(v, p); }
-__device__ __forceinline__ void b_store(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void b_store(gptr<double> p, double v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+__device__ __forceinline__ void b_store(gptr<float> p, float v)
+{
+    __builtin_nontemporal_store(v, p);
+}
 
-// One node of one step s of lattice L.
+// Moments, collision, outputs and store of node (x, y) whose pre-collision
+// populations (after its boundary closures) are f.
 template <typename T>
-__device__ __forceinline__ void batch_node(const BatchLat &L, int s, int n)
+__device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, int y, int n,
+                                             T f[Q], gptr<T> __restrict__ D, bool last)
+{
+    const unsigned F = L.flags;
+    const double rho0 = L.rho0;
+    const bool force = F & BF_FORCE;
+    const double om = gbl(L.om_col)[x];
+    double rl, ul, vl;   // next step's lagged moments as the reference's arrays hold them
+    if constexpr (sizeof(T) == 8) {
+        double *fd = reinterpret_cast<double *>(f);
+        const Moments m = moments_ref(fd, L.Fx, L.Fy);
+        rl = m.rho;
+        ul = m.ux;
+        vl = m.uy;
+        if (last && (F & BF_MOMENTS)) {
+            gbl(L.out[0])[n] = m.rho;
+            gbl(L.out[1])[n] = m.ux;
+            gbl(L.out[2])[n] = m.uy;
+        }
+        collide_ref(fd, m, om, force, L.Fx, L.Fy);
+    } else {
+        float *q = reinterpret_cast<float *>(f);
+        const float r0 = (float)rho0;
+        const MomentsF m = moments_f32(q, r0, (float)L.Fx, (float)L.Fy);
+        rl = (double)m.dr + rho0;
+        ul = (double)m.ux;
+        vl = (double)m.uy;
+        if (last && (F & BF_MOMENTS)) {
+            double fd[Q];
+#pragma unroll
+            for (int k = 0; k < Q; k++) fd[k] = b_dec(q[k], k, rho0);
+            const Moments md = moments_ref(fd, L.Fx, L.Fy);
+            gbl(L.out[0])[n] = md.rho;
+            gbl(L.out[1])[n] = md.ux;
+            gbl(L.out[2])[n] = md.uy;
+        }
+        collide_f32(q, m, r0, (float)om, force, (float)L.Fx, (float)L.Fy);
+    }
+
+    if (F & BF_LINER) {
+        if (y == 0 && x >= L.gap_x0 && x <= L.gap_x1) gbl(L.state)[8 * (x - L.gap_x0) + 6] = rl;
+        if (y == 1 && x >= L.gap_x0 && x <= L.gap_x1) {
+            gptr<double> sv = gbl(L.saved[(s + 1) & 1]) + 3 * (x - L.gap_x0);
+            sv[0] = rl;
+            sv[1] = ul;
+            sv[2] = vl;
+        }
+    }
+    for (int i = 0; i < L.nprobe; i++)
+        if (n == L.probe_n[i]) gbl(L.probe_hist)[(long long)(s + 1) * L.nprobe + i] = rl;
+
+    const unsigned P = L.plane;
+#pragma unroll
+    for (int k = 0; k < Q; k++) b_store(&D[k * P + (unsigned)n], f[k]);
+}
+
+// Node (x, y) of rows 1 .. Ny-2.  Away from the edge columns (2 <= x <= Nx-3,
+// 2 <= y <= Ny-3) every source x - c_k is an in-domain fluid node (walls occupy
+// only the outermost rows and columns): a plain gather.  Elsewhere the general
+// gather applies wrap, bounce-back and stale entries, and a convective end node
+// takes its inner neighbour's populations; no other closure touches these rows.
+// Every lane stores all nine slots (a solid node stores zeros: its slots are
+// never read), so a wavefront writes whole cache lines.
+template <typename T>
+__device__ __forceinline__ void batch_row_node(const BatchLat &L, gptr<const T> __restrict__ S,
+                                               gptr<T> __restrict__ D, int s, int x, int y,
+                                               bool last)
 {
     const int Nx = L.Nx, Ny = L.Ny;
-    const int y = n / Nx, x = n - y * Nx;
+    const int n = y * Nx + x;
+    const unsigned P = L.plane;
+    const unsigned F = L.flags;
+    T f[Q];
+    if ((unsigned)(x - 2) < (unsigned)(Nx - 4) && (unsigned)(y - 2) < (unsigned)(Ny - 4)) {
+#pragma unroll
+        for (int k = 0; k < Q; k++)
+            f[k] = S[k * P + (unsigned)(n - D2Q9::cy[k] * Nx - D2Q9::cx[k])];
+    } else {
+        if (b_solid(L, x, y)) {
+            if (last && (F & BF_MOMENTS)) {
+                gbl(L.out[0])[n] = NAN;
+                gbl(L.out[1])[n] = NAN;
+                gbl(L.out[2])[n] = NAN;
+            }
+#pragma unroll
+            for (int k = 0; k < Q; k++) b_store(&D[k * P + (unsigned)n], (T)0);
+            return;
+        }
+        int gx = x;
+        if ((F & BF_CONVECTIVE) && (x == 0 || x == Nx - 1)) gx = (x == 0) ? 1 : Nx - 2;
+        b_gather<T>(L, S, D, gx, y, f);
+    }
+    batch_finish<T>(L, s, x, y, n, f, D, last);
+}
+
+// Node of row 0 or Ny-1: wrap, walls, stale entries and the case's boundary
+// closures (moving lid, impedance wall, convective corners).
+template <typename T>
+__device__ __forceinline__ void batch_frame(const BatchLat &L, int s, int x, int y)
+{
+    const int Nx = L.Nx, Ny = L.Ny;
+    const int n = y * Nx + x;
     const unsigned F = L.flags;
     const bool last = (s + 1 == L.steps);
     if (b_solid(L, x, y)) {
         if (last && (F & BF_MOMENTS)) {
-            L.out[0][n] = NAN;
-            L.out[1][n] = NAN;
-            L.out[2][n] = NAN;
+            gbl(L.out[0])[n] = NAN;
+            gbl(L.out[1])[n] = NAN;
+            gbl(L.out[2])[n] = NAN;
         }
         return;
     }
-    const T *S = (const T *)L.buf[s & 1];
-    T *D = (T *)L.buf[(s + 1) & 1];
+    const gptr<const T> S = gbl((const T *)L.buf[s & 1]);
+    const gptr<T> D = gbl((T *)L.buf[(s + 1) & 1]);
     const double rho0 = L.rho0;
 
     // convective_{inlet,outlet}: the node takes the post-streaming populations of
@@ -184,12 +307,12 @@ __device__ __forceinline__ void batch_node(const BatchLat &L, int s, int n)
     const bool liner = (F & BF_LINER) && y == 0 && x >= L.gap_x0 && x <= L.gap_x1;
     if (liner) gy = 1;
     T f[Q];
-    b_gather(L, S, D, gx, gy, f);
+    b_gather<T>(L, S, D, gx, gy, f);
 
     if (liner) {
         // cases/nit/nit.c:13-47 per node, then velocity_bottom_regularized.c:22-70
         const int e = x - L.gap_x0;
-        double *st = L.state + 8 * e;
+        gptr<double> st = gbl(L.state) + 8 * e;
         TdibcState ts;
         ts.phi[0] = st[0];
         ts.phi[1] = st[1];
@@ -207,7 +330,7 @@ __device__ __forceinline__ void batch_node(const BatchLat &L, int s, int n)
         st[4] = ts.chi[0];
         st[5] = ts.chi[1];
         const double vbc = P.gain * (v / P.cu);
-        const double *lg = L.saved[s & 1] + 3 * e;
+        const gptr<const double> lg = gbl((const double *)L.saved[s & 1]) + 3 * e;
         double fi[Q], fb[Q];
 #pragma unroll
         for (int k = 0; k < Q; k++) fi[k] = b_dec(f[k], k, rho0);
@@ -224,83 +347,86 @@ __device__ __forceinline__ void batch_node(const BatchLat &L, int s, int n)
         f[7] = b_enc<T>(fd[7], 7, rho0);
         f[8] = b_enc<T>(fd[8], 8, rho0);
     }
-
-    const bool force = F & BF_FORCE;
-    const double om = L.om_col[x];
-    double rl, ul, vl;   // next step's lagged moments as the reference's arrays hold them
-    if constexpr (sizeof(T) == 8) {
-        double *fd = reinterpret_cast<double *>(f);
-        const Moments m = moments_ref(fd, L.Fx, L.Fy);
-        rl = m.rho;
-        ul = m.ux;
-        vl = m.uy;
-        if (last && (F & BF_MOMENTS)) {
-            L.out[0][n] = m.rho;
-            L.out[1][n] = m.ux;
-            L.out[2][n] = m.uy;
-        }
-        collide_ref(fd, m, om, force, L.Fx, L.Fy);
-    } else {
-        float *q = reinterpret_cast<float *>(f);
-        const float r0 = (float)rho0;
-        const MomentsF m = moments_f32(q, r0, (float)L.Fx, (float)L.Fy);
-        rl = (double)m.dr + rho0;
-        ul = (double)m.ux;
-        vl = (double)m.uy;
-        if (last && (F & BF_MOMENTS)) {
-            double fd[Q];
-#pragma unroll
-            for (int k = 0; k < Q; k++) fd[k] = b_dec(q[k], k, rho0);
-            const Moments md = moments_ref(fd, L.Fx, L.Fy);
-            L.out[0][n] = md.rho;
-            L.out[1][n] = md.ux;
-            L.out[2][n] = md.uy;
-        }
-        collide_f32(q, m, r0, (float)om, force, (float)L.Fx, (float)L.Fy);
-    }
-
-    if (F & BF_LINER) {
-        if (liner) L.state[8 * (x - L.gap_x0) + 6] = rl;
-        if (y == 1 && x >= L.gap_x0 && x <= L.gap_x1) {
-            double *sv = L.saved[(s + 1) & 1] + 3 * (x - L.gap_x0);
-            sv[0] = rl;
-            sv[1] = ul;
-            sv[2] = vl;
-        }
-    }
-    for (int i = 0; i < L.nprobe; i++)
-        if (n == L.probe_n[i]) L.probe_hist[(long long)(s + 1) * L.nprobe + i] = rl;
-
-    const unsigned P = L.plane;
-#pragma unroll
-    for (int k = 0; k < Q; k++) b_store(&D[k * P + (unsigned)n], f[k]);
+    batch_finish<T>(L, s, x, y, n, f, D, last);
 }
 
-// One time step of every lattice whose own step lies in [0, steps).  Persistent:
-// workgroup b owns tiles [t0, t1) of the tile table; with `banded` the blocks
-// that the hardware deals to one XCD (b % 8) own one contiguous band of tiles, so
-// a lattice's neighbouring rows meet in that XCD's L2.
-template <typename T>
-__global__ void __launch_bounds__(kBatchThreads) k_batch_step(const BatchLat *__restrict__ lats,
-                                                             const Tile *__restrict__ tiles,
-                                                             const BatchCtl *__restrict__ ctl,
-                                                             int local, int tile, int banded)
+// Node counts of the two tile kinds of a lattice (Nx, Ny >= 4): row tiles run over
+// whole rows 1 .. Ny-2, edge tiles over rows 0 and Ny-1.
+__host__ __device__ __forceinline__ int interior_nodes(int Nx, int Ny)
 {
-    const int ntiles = ctl->ntiles;
-    const long long epoch = ctl->epoch + local;
+    return Nx * (Ny - 2);
+}
+__host__ __device__ __forceinline__ int frame_nodes(int Nx, int Ny)
+{
+    return 2 * Nx;
+}
+constexpr int kFrameTile = 256;
+constexpr int kFrameThreads = 256;
+
+// Tile range of workgroup b out of G for n tiles.  mode 0: interleaved (tile t on
+// workgroup t mod G, so the chip sweeps the table as one front, as a hardware-
+// dispatched grid would); 1: one contiguous band per XCD (blocks b with equal b % 8
+// run on one XCD); 2: one contiguous range per workgroup.
+struct TileRange {
+    int t0, t1, dt;
+};
+__device__ __forceinline__ TileRange tile_range(int n, int mode)
+{
     const int G = gridDim.x, b = blockIdx.x;
-    const int chunk = banded ? (b % 8) * (G / 8) + b / 8 : b;
-    const int t0 = (int)((long long)ntiles * chunk / G);
-    const int t1 = (int)((long long)ntiles * (chunk + 1) / G);
-    for (int t = t0; t < t1; t++) {
+    if (mode == 0) return TileRange{b, n, G};
+    const int chunk = mode == 1 ? (b % 8) * (G / 8) + b / 8 : b;
+    return TileRange{(int)((long long)n * chunk / G), (int)((long long)n * (chunk + 1) / G), 1};
+}
+
+// One time step of every lattice whose own step lies in [0, steps): rows 1 .. Ny-2.
+// Persistent; `tile` consecutive nodes of those rows per tile.
+template <typename T>
+__global__ void __launch_bounds__(kBatchThreads) k_batch_interior(
+    const BatchLat *__restrict__ lats, const Tile *__restrict__ tiles,
+    const BatchCtl *__restrict__ ctl, int local, int tile, int mode)
+{
+    const long long epoch = ctl->epoch + local;
+    const TileRange R = tile_range(ctl->n_interior, mode);
+    for (int t = R.t0; t < R.t1; t += R.dt) {
         const Tile tl = tiles[t];
         const BatchLat &L = lats[tl.lat];
         const long long s = epoch - L.start;
         if (s < 0 || s >= L.steps) continue;
-        const int N = L.Nx * L.Ny;
-        const int end = min(tl.start + tile, N);
-        for (int n = tl.start + (int)threadIdx.x; n < end; n += (int)blockDim.x)
-            batch_node<T>(L, (int)s, n);
+        const int Nx = L.Nx;
+        const int end = min(tl.start + tile, interior_nodes(Nx, L.Ny));
+        const gptr<const T> S = gbl((const T *)L.buf[s & 1]);
+        const gptr<T> D = gbl((T *)L.buf[(s + 1) & 1]);
+        const bool last = (s + 1 == L.steps);
+        for (int m = tl.start + (int)threadIdx.x; m < end; m += (int)blockDim.x) {
+            // exact for m < 2^26: (m + 1/2)/Nx is >= 1/(2 Nx) away from an integer
+            const int r = (int)(((double)m + 0.5) * L.inv_nx);
+            batch_row_node<T>(L, S, D, (int)s, m - r * Nx, 1 + r, last);
+        }
+    }
+}
+
+// The same step for rows 0 and Ny-1.  Independent of k_batch_interior within a
+// step: the two write disjoint nodes and read only the previous step's buffer
+// (plus a node's own stale slots).
+template <typename T>
+__global__ void __launch_bounds__(kFrameThreads) k_batch_frame(
+    const BatchLat *__restrict__ lats, const Tile *__restrict__ tiles,
+    const BatchCtl *__restrict__ ctl, int local)
+{
+    const long long epoch = ctl->epoch + local;
+    const int base = ctl->n_interior;
+    const TileRange R = tile_range(ctl->n_frame, 0);
+    for (int t = R.t0; t < R.t1; t += R.dt) {
+        const Tile tl = tiles[base + t];
+        const BatchLat &L = lats[tl.lat];
+        const long long s = epoch - L.start;
+        if (s < 0 || s >= L.steps) continue;
+        const int Nx = L.Nx;
+        const int end = min(tl.start + kFrameTile, frame_nodes(Nx, L.Ny));
+        for (int f = tl.start + (int)threadIdx.x; f < end; f += (int)blockDim.x) {
+            const int top = f >= Nx;
+            batch_frame<T>(L, (int)s, f - top * Nx, top ? L.Ny - 1 : 0);
+        }
     }
 }
 
@@ -373,7 +499,7 @@ __global__ void __launch_bounds__(256) k_batch_init(const BatchLat *__restrict__
     if ((L.flags & BF_LINER) && x >= L.gap_x0 && x <= L.gap_x1) {
         const int e = x - L.gap_x0;
         if (y == 0) {
-            double *st = L.state + 8 * e;
+            gptr<double> st = gbl(L.state) + 8 * e;
             for (int i = 0; i < 6; i++) st[i] = 0.0;
             st[6] = rho;
             st[7] = 0.0;
