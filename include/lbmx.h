@@ -42,6 +42,7 @@ extern "C" {
 /* Storage / arithmetic of the populations. */
 #define LBMX_F64 0            /* FP64 populations, FP64 collision (reference arithmetic) */
 #define LBMX_F32_SHIFTED 1    /* FP32 populations stored as f_k - w_k*rho0, FP32 collision */
+#define LBMX_F32_WIDE 2       /* the same FP32 storage, collision evaluated in FP64 */
 
 /* Boundary-condition codes: one per BCFunction of include/lbm_bc.h. */
 #define LBMX_BC_CONVECTIVE_INLET 1          /* src/bc/convective_inlet.c */
@@ -74,11 +75,11 @@ typedef struct lbmx_lattice lbmx_lattice;
 
 typedef struct {
     int Nx, Ny;
-    int precision;        /* LBMX_F64 or LBMX_F32_SHIFTED */
+    int precision;        /* LBMX_F64, LBMX_F32_SHIFTED or LBMX_F32_WIDE */
     int per_x, per_y;     /* periodic wrap in streaming (lbm_core.h:23-24) */
     int use_ibb;          /* interpolated bounce-back at solid links (lbm_core.h:30) */
     double Fx, Fy;        /* Guo body force (lbm_core.h:25-26) */
-    double rho0;          /* shift density of LBMX_F32_SHIFTED storage (usually 1) */
+    double rho0;          /* shift density of the FP32 storage (usually 1) */
     unsigned flags;       /* LBMX_EXACT_STATE */
 } lbmx_config;
 

@@ -22,8 +22,9 @@
  *     velocity_top(Ny-1)                                     (velocity_top.c)
  *   moments        compute_macroscopic_fields.c:19-48
  * A case run in a batch gives, in LBMX_F64, bit for bit the result of the same
- * program run by the reference (and by lbmx_lattice); in LBMX_F32_SHIFTED, bit
- * for bit the result of an LBMX_F32_SHIFTED lbmx_lattice.  Results never depend
+ * program run by the reference (and by lbmx_lattice); in LBMX_F32_SHIFTED and
+ * LBMX_F32_WIDE, bit for bit the result of an lbmx_lattice of that precision.
+ * Results never depend
  * on the other lattices in the batch, on the launch configuration or on the
  * HIP-graph chunking.
  *
@@ -101,7 +102,7 @@ typedef struct {
 
 typedef struct lbmx_batch lbmx_batch;
 
-/* precision: LBMX_F64 or LBMX_F32_SHIFTED.  budget: bytes of device memory the
+/* precision: LBMX_F64, LBMX_F32_SHIFTED or LBMX_F32_WIDE.  budget: bytes of device memory the
  * batch may hold for its lattices (one arena, allocated here). */
 LBMX_API int lbmx_batch_create(int precision, size_t budget, lbmx_batch **out);
 LBMX_API void lbmx_batch_destroy(lbmx_batch *b);

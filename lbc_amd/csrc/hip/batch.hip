@@ -331,21 +331,25 @@ static int grid_of(const lbmx_batch *b)
     return std::max(8, (G / 8) * 8);
 }
 
-static int launch_step(lbmx_batch *b, int local)
+template <typename T, bool W>
+static void launch_step_t(lbmx_batch *b, int local)
 {
     const dim3 grid(grid_of(b)), block(b->tun.threads);
     const dim3 fgrid(2 * b->ncu), fblock(kFrameThreads);
-    if (b->precision == LBMX_F64) {
-        hipLaunchKernelGGL(k_batch_interior<double>, grid, block, 0, b->stream, b->lats_dev,
-                           b->tiles_dev, b->ctl_dev, local, b->tun.tile, b->tun.order);
-        hipLaunchKernelGGL(k_batch_frame<double>, fgrid, fblock, 0, b->stream, b->lats_dev,
-                           b->tiles_dev, b->ctl_dev, local);
-    } else {
-        hipLaunchKernelGGL(k_batch_interior<float>, grid, block, 0, b->stream, b->lats_dev,
-                           b->tiles_dev, b->ctl_dev, local, b->tun.tile, b->tun.order);
-        hipLaunchKernelGGL(k_batch_frame<float>, fgrid, fblock, 0, b->stream, b->lats_dev,
-                           b->tiles_dev, b->ctl_dev, local);
-    }
+    hipLaunchKernelGGL((k_batch_interior<T, W>), grid, block, 0, b->stream, b->lats_dev,
+                       b->tiles_dev, b->ctl_dev, local, b->tun.tile, b->tun.order);
+    hipLaunchKernelGGL((k_batch_frame<T, W>), fgrid, fblock, 0, b->stream, b->lats_dev,
+                       b->tiles_dev, b->ctl_dev, local);
+}
+
+static int launch_step(lbmx_batch *b, int local)
+{
+    if (b->precision == LBMX_F64)
+        launch_step_t<double, false>(b, local);
+    else if (b->precision == LBMX_F32_WIDE)
+        launch_step_t<float, true>(b, local);
+    else
+        launch_step_t<float, false>(b, local);
     HIPCHK(hipGetLastError());
     return LBMX_OK;
 }
@@ -362,9 +366,14 @@ static int launch_init(lbmx_batch *b, int slot)
     BatchSlot &S = b->slots[slot];
     const dim3 grid((S.c.Nx + 255) / 256, S.c.Ny), block(256);
     if (b->precision == LBMX_F64)
-        hipLaunchKernelGGL(k_batch_init<double>, grid, block, 0, b->stream, b->lats_dev, slot, S.init);
+        hipLaunchKernelGGL((k_batch_init<double, false>), grid, block, 0, b->stream, b->lats_dev,
+                           slot, S.init);
+    else if (b->precision == LBMX_F32_WIDE)
+        hipLaunchKernelGGL((k_batch_init<float, true>), grid, block, 0, b->stream, b->lats_dev,
+                           slot, S.init);
     else
-        hipLaunchKernelGGL(k_batch_init<float>, grid, block, 0, b->stream, b->lats_dev, slot, S.init);
+        hipLaunchKernelGGL((k_batch_init<float, false>), grid, block, 0, b->stream, b->lats_dev,
+                           slot, S.init);
     HIPCHK(hipGetLastError());
     return LBMX_OK;
 }
@@ -483,7 +492,7 @@ LBMX_API int lbmx_batch_create(int precision, size_t budget, lbmx_batch **out)
 {
     if (!out) return lbmx_fail(LBMX_ERR_ARG, "null argument");
     *out = nullptr;
-    if (precision != LBMX_F64 && precision != LBMX_F32_SHIFTED)
+    if (precision != LBMX_F64 && precision != LBMX_F32_SHIFTED && precision != LBMX_F32_WIDE)
         return lbmx_fail(LBMX_ERR_ARG, "unknown precision %d", precision);
     if (budget < ((size_t)1 << 20)) return lbmx_fail(LBMX_ERR_ARG, "budget below 1 MiB");
     lbmx_batch *b = new lbmx_batch();

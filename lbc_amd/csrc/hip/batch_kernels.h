@@ -181,7 +181,7 @@ __device__ __forceinline__ void b_store(gptr<float> p, float v)
 
 // Moments, collision, outputs and store of node (x, y) whose pre-collision
 // populations (after its boundary closures) are f.
-template <typename T>
+template <typename T, bool W>
 __device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, int y, int n,
                                              T f[Q], gptr<T> __restrict__ D, bool last)
 {
@@ -202,6 +202,22 @@ __device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, in
             gbl(L.out[2])[n] = m.uy;
         }
         collide_ref(fd, m, om, force, L.Fx, L.Fy);
+    } else if constexpr (W) {
+        float *q = reinterpret_cast<float *>(f);
+        const MomentsW m = moments_wide(q, rho0, L.Fx, L.Fy);
+        rl = rho0 + m.dr;
+        ul = m.ux;
+        vl = m.uy;
+        if (last && (F & BF_MOMENTS)) {
+            double fd[Q];
+#pragma unroll
+            for (int k = 0; k < Q; k++) fd[k] = b_dec(q[k], k, rho0);
+            const Moments md = moments_ref(fd, L.Fx, L.Fy);
+            gbl(L.out[0])[n] = md.rho;
+            gbl(L.out[1])[n] = md.ux;
+            gbl(L.out[2])[n] = md.uy;
+        }
+        collide_wide(q, m, rho0, om, force, L.Fx, L.Fy);
     } else {
         float *q = reinterpret_cast<float *>(f);
         const float r0 = (float)rho0;
@@ -245,7 +261,7 @@ __device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, in
 // takes its inner neighbour's populations; no other closure touches these rows.
 // Every lane stores all nine slots (a solid node stores zeros: its slots are
 // never read), so a wavefront writes whole cache lines.
-template <typename T>
+template <typename T, bool W>
 __device__ __forceinline__ void batch_row_node(const BatchLat &L, gptr<const T> __restrict__ S,
                                                gptr<T> __restrict__ D, int s, int x, int y,
                                                bool last)
@@ -274,12 +290,12 @@ __device__ __forceinline__ void batch_row_node(const BatchLat &L, gptr<const T> 
         if ((F & BF_CONVECTIVE) && (x == 0 || x == Nx - 1)) gx = (x == 0) ? 1 : Nx - 2;
         b_gather<T>(L, S, D, gx, y, f);
     }
-    batch_finish<T>(L, s, x, y, n, f, D, last);
+    batch_finish<T, W>(L, s, x, y, n, f, D, last);
 }
 
 // Node of row 0 or Ny-1: wrap, walls, stale entries and the case's boundary
 // closures (moving lid, impedance wall, convective corners).
-template <typename T>
+template <typename T, bool W>
 __device__ __forceinline__ void batch_frame(const BatchLat &L, int s, int x, int y)
 {
     const int Nx = L.Nx, Ny = L.Ny;
@@ -347,7 +363,7 @@ __device__ __forceinline__ void batch_frame(const BatchLat &L, int s, int x, int
         f[7] = b_enc<T>(fd[7], 7, rho0);
         f[8] = b_enc<T>(fd[8], 8, rho0);
     }
-    batch_finish<T>(L, s, x, y, n, f, D, last);
+    batch_finish<T, W>(L, s, x, y, n, f, D, last);
 }
 
 // Node counts of the two tile kinds of a lattice (Nx, Ny >= 4): row tiles run over
@@ -380,7 +396,7 @@ __device__ __forceinline__ TileRange tile_range(int n, int mode)
 
 // One time step of every lattice whose own step lies in [0, steps): rows 1 .. Ny-2.
 // Persistent; `tile` consecutive nodes of those rows per tile.
-template <typename T>
+template <typename T, bool W>
 __global__ void __launch_bounds__(kBatchThreads) k_batch_interior(
     const BatchLat *__restrict__ lats, const Tile *__restrict__ tiles,
     const BatchCtl *__restrict__ ctl, int local, int tile, int mode)
@@ -400,7 +416,7 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_interior(
         for (int m = tl.start + (int)threadIdx.x; m < end; m += (int)blockDim.x) {
             // exact for m < 2^26: (m + 1/2)/Nx is >= 1/(2 Nx) away from an integer
             const int r = (int)(((double)m + 0.5) * L.inv_nx);
-            batch_row_node<T>(L, S, D, (int)s, m - r * Nx, 1 + r, last);
+            batch_row_node<T, W>(L, S, D, (int)s, m - r * Nx, 1 + r, last);
         }
     }
 }
@@ -408,7 +424,7 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_interior(
 // The same step for rows 0 and Ny-1.  Independent of k_batch_interior within a
 // step: the two write disjoint nodes and read only the previous step's buffer
 // (plus a node's own stale slots).
-template <typename T>
+template <typename T, bool W>
 __global__ void __launch_bounds__(kFrameThreads) k_batch_frame(
     const BatchLat *__restrict__ lats, const Tile *__restrict__ tiles,
     const BatchCtl *__restrict__ ctl, int local)
@@ -425,7 +441,7 @@ __global__ void __launch_bounds__(kFrameThreads) k_batch_frame(
         const int end = min(tl.start + kFrameTile, frame_nodes(Nx, L.Ny));
         for (int f = tl.start + (int)threadIdx.x; f < end; f += (int)blockDim.x) {
             const int top = f >= Nx;
-            batch_frame<T>(L, (int)s, f - top * Nx, top ? L.Ny - 1 : 0);
+            batch_frame<T, W>(L, (int)s, f - top * Nx, top ? L.Ny - 1 : 0);
         }
     }
 }
@@ -438,7 +454,7 @@ static __global__ void k_batch_advance(BatchCtl *ctl, int n)
 // Initial state of one lattice: buf[1] = feq(M0) (what the reference's f_new holds
 // before the first step), buf[0] = collide(feq(M0)) with the lagged moments M0,
 // probe row 0, the impedance state and the first lagged row-1 moments.
-template <typename T>
+template <typename T, bool W>
 __global__ void __launch_bounds__(256) k_batch_init(const BatchLat *__restrict__ lats, int slot,
                                                     BatchInit in)
 {
@@ -483,6 +499,9 @@ __global__ void __launch_bounds__(256) k_batch_init(const BatchLat *__restrict__
         if constexpr (sizeof(T) == 8) {
             collide_ref(reinterpret_cast<double *>(q), Moments{rho, ux, uy}, om, force, L.Fx,
                         L.Fy);
+        } else if constexpr (W) {
+            collide_wide(reinterpret_cast<float *>(q), MomentsW{rho - rho0, ux, uy}, rho0, om,
+                         force, L.Fx, L.Fy);
         } else {
             MomentsF m;
             m.dr = (float)(rho - rho0);

@@ -131,14 +131,15 @@ extern "C" int lbmx_create(const lbmx_config *cfg, lbmx_lattice **out)
     if (cfg->Nx < 3 || cfg->Ny < 3)
         return lbmx_fail(LBMX_ERR_ARG, "lattice %dx%d: both sizes must be >= 3", cfg->Nx,
                          cfg->Ny);
-    if (cfg->precision != LBMX_F64 && cfg->precision != LBMX_F32_SHIFTED)
+    if (cfg->precision != LBMX_F64 && cfg->precision != LBMX_F32_SHIFTED &&
+        cfg->precision != LBMX_F32_WIDE)
         return lbmx_fail(LBMX_ERR_ARG, "unknown precision %d", cfg->precision);
     if ((long long)cfg->Nx * cfg->Ny > (1LL << 31) - (1LL << 24))
         return lbmx_fail(LBMX_ERR_ARG, "lattice too large for 32-bit node indexing");
 
     lbmx_lattice *L = new lbmx_lattice();
     L->cfg = *cfg;
-    if (L->cfg.precision == LBMX_F32_SHIFTED && L->cfg.rho0 == 0.0) L->cfg.rho0 = 1.0;
+    if (L->cfg.precision != LBMX_F64 && L->cfg.rho0 == 0.0) L->cfg.rho0 = 1.0;
     if (L->cfg.precision == LBMX_F64) L->cfg.rho0 = 0.0;
     L->Nx = cfg->Nx;
     L->Ny = cfg->Ny;
@@ -481,6 +482,7 @@ Geo lbmx_geo(const lbmx_lattice *L)
     g.per_x = L->cfg.per_x;
     g.per_y = L->cfg.per_y;
     g.rho0 = L->cfg.rho0;
+    g.wide = L->cfg.precision == LBMX_F32_WIDE;
     return g;
 }
 

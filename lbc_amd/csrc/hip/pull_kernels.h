@@ -63,6 +63,10 @@ __global__ void __launch_bounds__(kPullThreads) k_pull(const T *__restrict__ S, 
         double *f = reinterpret_cast<double *>(p);
         const Moments m = moments_ref(f, a.s.Fx_lag, a.s.Fy_lag);
         collide_ref(f, m, g.om, FORCE, a.s.Fx, a.s.Fy);
+    } else if (g.wide) {
+        float *q = reinterpret_cast<float *>(p);
+        const MomentsW m = moments_wide(q, g.rho0, a.s.Fx_lag, a.s.Fy_lag);
+        collide_wide(q, m, g.rho0, g.om, FORCE, a.s.Fx, a.s.Fy);
     } else {
         float *q = reinterpret_cast<float *>(p);
         const float r0 = (float)g.rho0;
@@ -91,6 +95,11 @@ __global__ void __launch_bounds__(256) k_collide_inplace(T *__restrict__ S, Pull
         const Moments m = a.s.lag_arrays ? Moments{a.F.R[n], a.F.U[n], a.F.V[n]}
                                          : moments_ref(f, a.s.Fx_lag, a.s.Fy_lag);
         collide_ref(f, m, g.om, a.s.force, a.s.Fx, a.s.Fy);
+    } else if (g.wide) {
+        float *q = reinterpret_cast<float *>(p);
+        const MomentsW m = a.s.lag_arrays ? MomentsW{a.F.R[n] - g.rho0, a.F.U[n], a.F.V[n]}
+                                          : moments_wide(q, g.rho0, a.s.Fx_lag, a.s.Fy_lag);
+        collide_wide(q, m, g.rho0, g.om, a.s.force, a.s.Fx, a.s.Fy);
     } else {
         float *q = reinterpret_cast<float *>(p);
         const float r0 = (float)g.rho0;

@@ -49,6 +49,7 @@ struct Geo {
     double om;             // uniform omega (geom 0)
     int per_x, per_y;
     double rho0;           // shift density of FP32 storage
+    int wide;              // FP32 storage with FP64 arithmetic (LBMX_F32_WIDE)
 };
 
 struct StepFlags {
@@ -145,6 +146,72 @@ __device__ __forceinline__ void collide_f32(float g[Q], const MomentsF &m, float
     }
 }
 
+// ---- FP64 arithmetic on FP32-stored deviations (LBMX_F32_WIDE) ---------------
+// Same storage and the same formulas as above, evaluated in FP64 and rounded to
+// FP32 once per stored value.  MI355X's FP64 vector rate equals its non-packed
+// FP32 rate, so in this memory-bound step the wider arithmetic costs no time,
+// while it removes the FP32 rounding of the equilibrium, whose error (~1e-8 of
+// g_eq per step, correlated from step to step) otherwise accumulates into the
+// slowly varying non-equilibrium part.
+struct MomentsW {
+    double dr, ux, uy;
+};
+
+__device__ __forceinline__ MomentsW moments_wide(const float g[Q], double rho0, double Fx,
+                                                 double Fy)
+{
+    double d[Q];
+#pragma unroll
+    for (int k = 0; k < Q; k++) d[k] = (double)g[k];
+    const double dr = ((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
+    const double jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
+    const double jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
+    double rho = rho0 + dr;
+    if (rho < kRhoFloor) rho = kRhoFloor;
+    MomentsW m;
+    m.dr = rho - rho0;
+    m.ux = (jx + 0.5 * Fx) / rho;
+    m.uy = (jy + 0.5 * Fy) / rho;
+    return m;
+}
+
+__device__ __forceinline__ void collide_wide(float g[Q], const MomentsW &m, double rho0, double om,
+                                             bool force, double Fx, double Fy)
+{
+    const double rho = rho0 + m.dr;
+    const double usq15 = 1.5 * (m.ux * m.ux + m.uy * m.uy);
+    const double uF3 = 3.0 * (m.ux * Fx + m.uy * Fy);
+    const double gf = 1.0 - 0.5 * om;
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const double w = D2Q9::w[k];
+        const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
+        const double cu = (cx == 0 ? 0.0 : (cx > 0 ? m.ux : -m.ux)) +
+                          (cy == 0 ? 0.0 : (cy > 0 ? m.uy : -m.uy));
+        const double geq = w * (m.dr + rho * (cu * (3.0 + 4.5 * cu) - usq15));
+        const double gk = (double)g[k];
+        double r = gk - om * (gk - geq);
+        if (force) {
+            const double cF = (cx == 0 ? 0.0 : (cx > 0 ? Fx : -Fx)) +
+                              (cy == 0 ? 0.0 : (cy > 0 ? Fy : -Fy));
+            r = r + gf * w * (3.0 * cF + 9.0 * cu * cF - uF3);
+        }
+        g[k] = (float)r;
+    }
+}
+
+// Lagged moments in the reference's FP64 form from either FP32 arithmetic.
+__device__ __forceinline__ Moments moments_store(const float q[Q], double rho0, double Fx,
+                                                 double Fy, bool wide)
+{
+    if (wide) {
+        const MomentsW m = moments_wide(q, rho0, Fx, Fy);
+        return Moments{rho0 + m.dr, m.ux, m.uy};
+    }
+    const MomentsF m = moments_f32(q, (float)rho0, (float)Fx, (float)Fy);
+    return Moments{(double)m.dr + rho0, (double)m.ux, (double)m.uy};
+}
+
 // ---- lagged moments and post-collision values on demand --------------------
 __device__ __forceinline__ Moments lagged_at(const double *S, const Geo &g, const Fields &F,
                                              const StepFlags &s, size_t n)
@@ -165,8 +232,7 @@ __device__ __forceinline__ Moments lagged_at(const float *S, const Geo &g, const
     float q[Q];
 #pragma unroll
     for (int k = 0; k < Q; k++) q[k] = S[k * g.plane + n];
-    MomentsF m = moments_f32(q, (float)g.rho0, (float)s.Fx_lag, (float)s.Fy_lag);
-    return Moments{(double)m.dr + g.rho0, (double)m.ux, (double)m.uy};
+    return moments_store(q, g.rho0, s.Fx_lag, s.Fy_lag, g.wide);
 }
 
 // Post-collision populations of node n (reference values, FP64), i.e. what the
@@ -189,16 +255,22 @@ __device__ __forceinline__ void star_node(const float *S, const Geo &g, const Fi
 #pragma unroll
     for (int k = 0; k < Q; k++) q[k] = S[k * g.plane + n];
     if (!(s.exact || solid_at(g, n))) {
-        const float r0 = (float)g.rho0;
-        MomentsF m;
-        if (s.lag_arrays) {
-            m.dr = (float)(F.R[n] - g.rho0);
-            m.ux = (float)F.U[n];
-            m.uy = (float)F.V[n];
+        if (g.wide) {
+            const MomentsW m = s.lag_arrays ? MomentsW{F.R[n] - g.rho0, F.U[n], F.V[n]}
+                                            : moments_wide(q, g.rho0, s.Fx_lag, s.Fy_lag);
+            collide_wide(q, m, g.rho0, omega_at(g, n), s.force, s.Fx, s.Fy);
         } else {
-            m = moments_f32(q, r0, (float)s.Fx_lag, (float)s.Fy_lag);
+            const float r0 = (float)g.rho0;
+            MomentsF m;
+            if (s.lag_arrays) {
+                m.dr = (float)(F.R[n] - g.rho0);
+                m.ux = (float)F.U[n];
+                m.uy = (float)F.V[n];
+            } else {
+                m = moments_f32(q, r0, (float)s.Fx_lag, (float)s.Fy_lag);
+            }
+            collide_f32(q, m, r0, (float)omega_at(g, n), s.force, (float)s.Fx, (float)s.Fy);
         }
-        collide_f32(q, m, r0, (float)omega_at(g, n), s.force, (float)s.Fx, (float)s.Fy);
     }
 #pragma unroll
     for (int k = 0; k < Q; k++) out[k] = (double)q[k] + D2Q9::w[k] * g.rho0;
@@ -273,6 +345,20 @@ __global__ void __launch_bounds__(kBulkThreads) k_bulk(T *__restrict__ S, T *__r
                 }
             }
             collide_ref(f, m, om, a.s.force, a.s.Fx, a.s.Fy);
+        } else if (g.wide) {
+            float *q = reinterpret_cast<float *>(p);
+            MomentsW m;
+            if (a.s.lag_arrays) {
+                m = MomentsW{a.F.R[n] - g.rho0, a.F.U[n], a.F.V[n]};
+            } else {
+                m = moments_wide(q, g.rho0, a.s.Fx_lag, a.s.Fy_lag);
+                if (a.s.exact) {
+                    a.F.R[n] = g.rho0 + m.dr;
+                    a.F.U[n] = m.ux;
+                    a.F.V[n] = m.uy;
+                }
+            }
+            collide_wide(q, m, g.rho0, om, a.s.force, a.s.Fx, a.s.Fy);
         } else {
             float *q = reinterpret_cast<float *>(p);
             const float r0 = (float)g.rho0;

@@ -13,7 +13,8 @@ import numpy as np
 from . import lib
 
 F64 = 0
-F32_SHIFTED = 1
+F32_SHIFTED = 1      # FP32 storage of f - w*rho0, FP32 collision
+F32_WIDE = 2         # the same storage, collision in FP64
 EXACT_STATE = 1
 
 BC_CODES = {

@@ -130,28 +130,32 @@ def test_gather_form_bit_identical(sc, chunks):
     assert same_bits(rho, ref.rho) and same_bits(u, ref.u) and same_bits(v, ref.v)
 
 
+@pytest.mark.parametrize("precision", [1, 2], ids=["f32", "f32wide"])
 @pytest.mark.parametrize("sc", GATHER_CASES, ids=[s.name for s in GATHER_CASES])
-def test_gather_form_fp32_equals_push(sc):
-    """FP32-shifted: gather form and push form round identically."""
+def test_gather_form_fp32_equals_push(sc, precision):
+    """FP32 storage (FP32 or FP64 arithmetic): gather form and push form round
+    identically."""
     state = sc.initial_state()
-    a = _gather_run(sc, state, (8,), precision=1)
-    b = _gather_run(sc, state, (1,) * 8, precision=1)
+    a = _gather_run(sc, state, (8,), precision=precision)
+    b = _gather_run(sc, state, (1,) * 8, precision=precision)
     for x, y in zip(a, b):
         assert same_bits(x, y)
 
 
-# FP32-shifted storage against the FP64 oracle: relative L2 error of the
-# velocity after NSTEPS (north star: <= 1e-5 vs FP64).
-FP32_VEL_L2 = 1e-5
+# FP32 storage against the FP64 oracle: relative L2 error of the velocity after
+# NSTEPS.  Bars: 1e-5 with FP32 arithmetic (LBMX_F32_SHIFTED), 1e-6 with FP64
+# arithmetic on the FP32-stored deviations (LBMX_F32_WIDE).
+FP32_VEL_L2 = {1: 1e-5, 2: 1e-6}
 
 
+@pytest.mark.parametrize("precision", [1, 2], ids=["f32", "f32wide"])
 @pytest.mark.parametrize("sc", SCENARIOS[:16], ids=IDS[:16])
-def test_fp32_shifted_close_to_fp64(sc):
+def test_fp32_shifted_close_to_fp64(sc, precision):
     state = sc.initial_state()
     ref, ref_forces = run_oracle(sc, state, NSTEPS)
-    got = run_gpu(sc, state, NSTEPS, precision=1)
+    got = run_gpu(sc, state, NSTEPS, precision=precision)
     fluid = sc.mask == 0
     du = np.sqrt(np.sum((got["u"] - ref.u)[fluid] ** 2 + (got["v"] - ref.v)[fluid] ** 2))
     nu = np.sqrt(np.sum(ref.u[fluid] ** 2 + ref.v[fluid] ** 2))
-    assert du / nu <= FP32_VEL_L2, du / nu
+    assert du / nu <= FP32_VEL_L2[precision], du / nu
     assert rel_err(got["rho"], ref.rho, fluid) <= 1e-6
