@@ -155,7 +155,8 @@ typedef struct {
     int nfreq;                 /* <= 64 frequencies, linear from f_lo to f_hi (cycles/step) */
     double f_lo, f_hi;
     int up_probe, down_probe;  /* probe indices of the case */
-    int up_window;             /* samples of the upstream probe used (incident pulse) */
+    int up_t0, up_t1;          /* samples [t0, t1) of the upstream probe (incident pulse) */
+    int down_t0, down_t1;      /* samples [t0, t1) of the downstream probe (transmitted) */
 } lbmx_tl_spec;
 
 typedef struct {

@@ -1,9 +1,10 @@
 """ctypes view of the batched engine and the request server (include/lbmx_batch.h).
 
-:class:`Case` describes one lattice program; :meth:`Case.reference_program` states
-the same program in the reference's own terms (solid mask, per-node omega,
-boundary table in application order, initial moment arrays), which is what the
-oracle and the single-lattice API run for comparison.  :class:`Batch` and
+:class:`Case` describes one lattice program; its methods ``solid_mask``,
+``omega_column``, ``boundary_table`` and ``initial_fields`` state the same
+program in the reference's own terms (solid mask, per-node omega, boundary table
+in application order, initial moment arrays), which is what the oracle and the
+single-lattice API (``single_lattice``) run for comparison.  :class:`Batch` and
 :class:`Server` wrap the C objects.
 """
 from __future__ import annotations
@@ -54,7 +55,8 @@ class Tuning(C.Structure):
 
 class TlSpec(C.Structure):
     _fields_ = [("nfreq", C.c_int), ("f_lo", C.c_double), ("f_hi", C.c_double),
-                ("up_probe", C.c_int), ("down_probe", C.c_int), ("up_window", C.c_int)]
+                ("up_probe", C.c_int), ("down_probe", C.c_int), ("up_t0", C.c_int),
+                ("up_t1", C.c_int), ("down_t0", C.c_int), ("down_t1", C.c_int)]
 
 
 class Result(C.Structure):

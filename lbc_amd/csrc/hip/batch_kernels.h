@@ -532,16 +532,16 @@ __global__ void __launch_bounds__(256) k_batch_init(const BatchLat *__restrict__
 }
 
 // ---- spectra of a finished request's probes (server) ------------------------------
-// Block (f, p): P(f) = sum_t p'(t) e^{-2 pi i f t} over the first `rows[p]`
-// samples of probe `probe[p]`, p' = cs2 (rho - rho0).  Fixed-order reduction, so
-// the result does not depend on anything but the history.
+// Block (f, p): P(f) = sum_t p'(t) e^{-2 pi i f t} over samples [t0[p], t1[p]) of
+// probe `probe[p]`, p' = cs2 (rho - rho0).  Fixed-order reduction, so the result
+// does not depend on anything but the history.
 constexpr int kSpecThreads = 256;
 
 struct SpecArgs {
     const double *hist;
     int nprobe;
     int probe[2];
-    int rows[2];
+    int t0[2], t1[2];
     double rho0, cs2, f_lo, df;
     double *out;             // nfreq x 2 x (re, im)
 };
@@ -552,7 +552,7 @@ static __global__ void __launch_bounds__(kSpecThreads) k_spectrum(SpecArgs a)
     const int fi = blockIdx.x, p = blockIdx.y;
     const double f = a.f_lo + a.df * fi;
     double re = 0.0, im = 0.0;
-    for (int t = threadIdx.x; t < a.rows[p]; t += kSpecThreads) {
+    for (int t = a.t0[p] + (int)threadIdx.x; t < a.t1[p]; t += kSpecThreads) {
         const double v = a.cs2 * (a.hist[(long long)t * a.nprobe + a.probe[p]] - a.rho0);
         double sn, cs;
         sincospi(2.0 * f * (double)t, &sn, &cs);

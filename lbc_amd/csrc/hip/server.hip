@@ -78,8 +78,9 @@ static int validate_tl(const lbmx_case *c, const lbmx_tl_spec *tl)
     if (tl->up_probe < 0 || tl->up_probe >= c->nprobe || tl->down_probe < 0 ||
         tl->down_probe >= c->nprobe)
         return lbmx_fail(LBMX_ERR_ARG, "spectrum probes must name probes of the case");
-    if (tl->up_window < 1 || tl->up_window > c->steps)
-        return lbmx_fail(LBMX_ERR_ARG, "up_window must be in [1, steps]");
+    if (tl->up_t0 < 0 || tl->up_t1 <= tl->up_t0 || tl->up_t1 > c->steps + 1 ||
+        tl->down_t0 < 0 || tl->down_t1 <= tl->down_t0 || tl->down_t1 > c->steps + 1)
+        return lbmx_fail(LBMX_ERR_ARG, "probe windows must lie inside [0, steps]");
     return LBMX_OK;
 }
 
@@ -94,8 +95,10 @@ static int finish(lbmx_server *s, const Running &R, lbmx_result *out)
     a.nprobe = S->c.nprobe;
     a.probe[0] = tl.up_probe;
     a.probe[1] = tl.down_probe;
-    a.rows[0] = tl.up_window;
-    a.rows[1] = S->c.steps;
+    a.t0[0] = tl.up_t0;
+    a.t1[0] = tl.up_t1;
+    a.t0[1] = tl.down_t0;
+    a.t1[1] = tl.down_t1;
     a.rho0 = S->c.rho0;
     const double cs = 1.0 / std::sqrt(3.0);
     a.cs2 = cs * cs;
