@@ -164,6 +164,87 @@ def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True):
     }
 
 
+def serving_bench(rates=(10, 30, 100, 300, 1000), n=32, seq_sample=6, seed=2026,
+                  budget_gib=200):
+    """The serving loop: n seeded liner-duct requests (Nx in [1024, 8192], Ny in
+    [64, 256], 2,000-20,000 steps, seeded impedance) arriving as a Poisson stream
+    at each rate, admitted by a continuous-batching server (FP32 storage, FP64
+    collision) under an HBM budget; each returns a TL spectrum.  Throughput =
+    requests / (last completion - first arrival).  Baseline: a seeded sample of the
+    same requests solved one after another with plain single-lattice FP64 launches
+    (stepping time only), which also gives the served spectra's TL error."""
+    from lbc_amd import batch as B
+    from lbc_amd import native
+    from lbc_amd import serving as SV
+    prec = native.F32_WIDE
+    reqs = SV.draw_requests(n, seed=seed)
+    # launch configuration tuned once on a representative batch
+    tb = B.Batch(prec, budget=sum(B.case_bytes(r.case(), prec) for r in reqs[:16]) + (64 << 20))
+    for r in reqs[:16]:
+        tb.add(r.case())
+    tuning = tb.tune(trial_steps=3)
+    tb.close()
+    try:
+        server = B.Server(prec, budget=budget_gib << 30, chunk=64, tuning=tuning)
+        budget = budget_gib
+    except Exception as e:   # a box with less free HBM: report the budget actually used
+        log(f"server with a {budget_gib} GiB budget failed ({e}); using 64 GiB")
+        server = B.Server(prec, budget=64 << 30, chunk=64, tuning=tuning)
+        budget = 64
+    sweep = []
+    served = {}
+    for rate in rates:
+        arrivals = SV.poisson_arrivals(n, rate, seed=seed + int(rate))
+        res, wall = SV.serve(server, reqs, arrivals)
+        bad = [r.status for r in res.values() if r.status != 0]
+        t0 = min(r.t_submit for r in res.values())
+        t1 = max(r.t_done for r in res.values())
+        lat = SV.latency_summary(res)
+        row = {"rate_per_s": rate, "requests": n, "solves_per_s": round(n / (t1 - t0), 3),
+               "p50_s": round(lat["p50_s"], 4), "p95_s": round(lat["p95_s"], 4),
+               "p99_s": round(lat["p99_s"], 4), "mean_s": round(lat["mean_s"], 4),
+               "failed": len(bad)}
+        sweep.append(row)
+        served = res
+        log(f"serving rate {rate}/s: {row}")
+    peak = server.stats()["peak_bytes"]
+    server.close()
+
+    rng = np.random.default_rng(seed + 1)
+    pick = sorted(rng.choice(n, size=min(seq_sample, n), replace=False).tolist())
+    seq_t, tl_err = [], []
+    f = np.linspace(SV.F_LO, SV.F_HI, SV.NFREQ)
+    for i in pick:
+        r = reqs[i]
+        lat = r.case().single_lattice(native.F64, record_steps=r.steps + 1)
+        lat.synchronize()
+        t1 = time.perf_counter()
+        lat.step(r.steps)
+        lat.synchronize()
+        seq_t.append(time.perf_counter() - t1)
+        hist = lat.probe_history()
+        lat.close()
+        tl64, up64, _ = SV.tl_from_history(hist, r)
+        tl32 = np.array(served[i].tl_db[:SV.NFREQ])
+        band = up64 > 1e-2 * up64.max()
+        tl_err.append(float(np.abs(tl32 - tl64)[band].max()))
+    seq_rate = 1.0 / float(np.mean(seq_t))
+    sat = max(row["solves_per_s"] for row in sweep)
+    log(f"sequential FP64 sample {len(pick)}: {np.mean(seq_t):.3f} s per solve "
+        f"({seq_rate:.2f} solves/s); served max {sat:.2f} solves/s ({sat / seq_rate:.2f}x); "
+        f"TL difference vs FP64 max {max(tl_err):.2e} dB")
+    return {
+        "workload": f"serving_liner_requests_{n}_per_rate_fp32_storage_fp64_collision",
+        "budget_GiB": budget, "peak_bytes": int(peak), "chunk_steps": 64,
+        "tuning": {"threads": tuning[0], "tile": tuning[1], "groups_per_cu": tuning[2],
+                   "order": tuning[3]},
+        "sweep": sweep,
+        "sequential_fp64_solves_per_s": round(seq_rate, 3), "sequential_sample": len(pick),
+        "speedup_vs_sequential_at_saturation": round(sat / seq_rate, 2),
+        "tl_max_abs_diff_db_vs_fp64": float(f"{max(tl_err):.3e}"),
+    }
+
+
 def characteristic_velocity(c):
     """Velocity scale of a batched case: the wall speed (Couette, cavity), the
     Taylor-Green amplitude, or for the force-driven channel the smaller of the
@@ -241,6 +322,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=5000)
     ap.add_argument("--batched-sample", type=int, default=48,
                     help="lattices timed one by one for the sequential FP64 baseline")
+    ap.add_argument("--serving", type=int, default=32, metavar="N",
+                    help="requests per arrival rate of the serving sweep (0: skip)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run one plain device-to-device copy of GIB GiB (torch) and "
                          "exit: the known-size reference for scaling rocprofv3 "
@@ -297,6 +380,8 @@ def main():
     if args.batched > 0:
         secondary["batched"] = batched_bench(args.batched, args.batched_steps,
                                              args.batched_sample)
+    if args.serving > 0:
+        secondary["serving"] = serving_bench(n=args.serving)
 
     cpu = None
     if not args.no_cpu:

@@ -1,0 +1,110 @@
+"""CPU checks of the serving loop's host side (no GPU calls).
+
+* Seeded requests follow the stated distributions and are valid cases.
+* The impedance ODE of every request is stable under the reference's in-place
+  update (nit.c:31), checked on the ODE itself and on its amplification matrix.
+* The spectrum windows keep the incident and transmitted pulses and end before
+  the first reflection can arrive.
+* The host DFT used to check the device spectra is the DFT: a Gaussian pulse's
+  spectrum against the closed form.
+* Poisson arrivals have the requested rate.
+"""
+import math
+
+import numpy as np
+
+from lbc_amd import batch as B
+from lbc_amd import serving as SV
+
+
+def test_request_distribution():
+    rs = SV.draw_requests(2000, seed=1)
+    nx = np.array([r.Nx for r in rs])
+    ny = np.array([r.Ny for r in rs])
+    st = np.array([r.steps for r in rs])
+    assert nx.min() >= 1024 and nx.max() <= 8192 and (nx % 32 == 0).all()
+    assert ny.min() >= 64 and ny.max() <= 256 and (ny % 32 == 0).all()
+    assert st.min() >= 2000 and st.max() <= 20000
+    assert abs(st.mean() - 11000) < 400
+    R = np.array([r.R for r in rs])
+    fr = np.array([r.f_res for r in rs])
+    amp = np.array([r.amp for r in rs])
+    assert R.min() >= 0.5 and R.max() <= 3.0
+    assert fr.min() >= 1 / 80 and fr.max() <= 1 / 20
+    assert amp.min() >= 1e-4 and amp.max() <= 1e-3
+    cells = (nx * ny).mean()
+    assert 6.5e5 < cells < 8.5e5           # ~0.74M cells per request on average
+    for r in rs[:40]:
+        B.case_check(r.case())
+    assert [(r.Nx, r.R) for r in SV.draw_requests(5, seed=1)] == [(r.Nx, r.R) for r in rs[:5]]
+
+
+def euler_growth(imp, steps=4000):
+    """Largest |state| after driving the explicit-Euler ODE (nit.c:13-47) with a
+    unit impulse of pressure and then nothing."""
+    phi, psi, chi = [0.0, 0.0], [0.0, 0.0], [0.0, 0.0]
+    peak = 0.0
+    for t in range(steps):
+        p = 1.0 if t == 0 else 0.0
+        for i in range(2):
+            phi[i] = phi[i] + (p + imp["lambdas"][i] * phi[i])
+            psi[i] = psi[i] + (p + imp["alpha"][i] * psi[i] - imp["beta"][i] * chi[i])
+            chi[i] = chi[i] + (imp["beta"][i] * psi[i] + imp["alpha"][i] * chi[i])
+        if t > steps - 100:
+            peak = max(peak, abs(psi[0]) + abs(chi[0]))
+    return peak
+
+
+def test_every_request_ode_is_stable():
+    for r in SV.draw_requests(200, seed=4):
+        imp = r.impedance()
+        assert euler_growth(imp) < 1.0, r
+
+
+def test_in_place_step_amplification():
+    """nit.c:31 updates chi with the new psi: one step of the homogeneous pair is
+    [[1+a, -b], [b(1+a), 1+a-b^2]], whose determinant is (1+a)^2, so the
+    impulse response decays like |1+a|^t for every damped pole a +- ib, even
+    where a plain explicit Euler step (|1+a+ib| > 1) would grow."""
+    r = SV.LinerRequest(1024, 64, 0.5, 1 / 20, 16.0, 5e-4, 2000)
+    imp = r.impedance()
+    a, b = imp["alpha"][0], imp["beta"][0]
+    assert (1 + a) ** 2 + b ** 2 > 1.0                   # plain explicit Euler: unstable
+    M = np.array([[1 + a, -b], [b * (1 + a), 1 + a - b * b]])
+    assert abs(np.linalg.det(M) - (1 + a) ** 2) < 1e-14
+    assert euler_growth(imp) < 1e-3
+
+
+def test_windows_follow_the_pulse():
+    for r in SV.draw_requests(50, seed=5):
+        (u0, u1), (d0, d1) = r.windows()
+        c = SV.CS
+        t_inc = (r.Nx / 4 - r.Nx / 8) / c                       # incident peak upstream
+        t_refl = ((3 * r.Nx / 8 - r.Nx / 8) + (3 * r.Nx / 8 - r.Nx / 4)) / c   # liner echo
+        t_tr = (3 * r.Nx / 4 - r.Nx / 8) / c                      # transmitted peak
+        t_end = (r.Nx / 8 + 3 * r.Nx / 4) / c                     # inlet echo downstream
+        assert u0 == 0 and u1 <= t_refl + 1 and u1 > t_inc
+        if r.steps > t_tr + t_inc:
+            assert d0 < t_tr < d1 <= t_end and abs((d1 - d0) - (u1 - u0)) <= 1
+        assert 0 <= d0 < d1 <= r.steps
+
+
+def test_host_spectrum_of_a_gaussian():
+    # p'(t) = A exp(-(t - t0)^2 / (2 s^2)): |P(f)| = A s sqrt(2 pi) exp(-2 pi^2 s^2 f^2)
+    A, t0, s = 2e-5, 300.0, 9.0
+    t = np.arange(800)
+    rho = 1.0 + A * np.exp(-(t - t0) ** 2 / (2 * s * s)) / (SV.CS * SV.CS)
+    f = np.linspace(SV.F_LO, SV.F_HI, SV.NFREQ)
+    P = SV.spectrum(rho, (0, 800), f)
+    exact = A * s * math.sqrt(2 * math.pi) * np.exp(-2 * math.pi ** 2 * s * s * f * f)
+    assert np.abs(np.abs(P) - exact).max() < 1e-9 * exact.max()
+    # shifting the window does not change the magnitude
+    P2 = SV.spectrum(np.concatenate([np.ones(50), rho]), (50, 850), f)
+    assert np.allclose(np.abs(P2), np.abs(P), rtol=1e-12, atol=0)
+
+
+def test_poisson_rate():
+    for rate in (10.0, 1000.0):
+        a = SV.poisson_arrivals(20000, rate, seed=3)
+        assert np.all(np.diff(a) > 0)
+        assert abs(len(a) / a[-1] / rate - 1.0) < 0.03
