@@ -15,8 +15,11 @@ Also reported in the same JSON line:
   cpu_baseline  the reference library (oracle/_ref, compiled from lbc's own
                 sources with its flags) timed on this host through main_lbm on a
                 bounded sample of the same workload
-  secondary     FP32-shifted MLUPS at 16384^2 and the FP64 cylinder channel
-                2048 x 512 (interpolated bounce-back, regularized inlet/outlet)
+  secondary     FP32-storage MLUPS at 16384^2 (FP32 collision and FP64 collision),
+                the FP64 cylinder channel 2048 x 512 (interpolated bounce-back,
+                regularized inlet/outlet), the batched workload (1,024 lattices x
+                5,000 steps) and the serving sweep (Poisson arrivals of liner-duct
+                requests), each against sequential single-lattice FP64 solves
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--size N]
 """
@@ -88,23 +91,25 @@ def cylinder_lattice(precision):
     return lat
 
 
-def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True):
+def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True, precision=None):
     """The batched workload: n independent lattices (Poiseuille / Couette / cavity /
     Taylor-Green, sizes uniform in {64..1024} x {64..512}) advanced `steps` steps in
-    FP32-shifted storage, one persistent HIP-graph-captured launch per step.
+    FP32-shifted storage with the collision in FP64 (LBMX_F32_WIDE), one persistent
+    HIP-graph-captured launch pair per step.
     Baseline: the same lattices one after another on the single-lattice FP64 path
     (its plain per-step launches), timed on a seeded random sample of `sample`
     lattices and scaled by n/sample (an unbiased estimate of the full sequential
     time); the sample also gives the FP32-vs-FP64 velocity error."""
     from lbc_amd import batch as B
     from lbc_amd import native
+    prec = native.F32_WIDE if precision is None else precision
     cases = B.batched_workload(n, steps=steps, seed=seed)
     rng = np.random.default_rng(seed + 1)
     pick = sorted(rng.choice(n, size=min(sample, n), replace=False).tolist())
     for i, c in enumerate(cases):
         c.keep_moments = i in pick
-    budget = sum(B.case_bytes(c, native.F32_SHIFTED) for c in cases) + (256 << 20)
-    b = B.Batch(native.F32_SHIFTED, budget=budget, chunk=32)
+    budget = sum(B.case_bytes(c, prec) for c in cases) + (256 << 20)
+    b = B.Batch(prec, budget=budget, chunk=32)
     ids = [b.add(c) for c in cases]
     t0 = time.perf_counter()
     tuning = b.tune(trial_steps=3) if tune else b.tuning()
@@ -116,7 +121,8 @@ def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True):
     wall = time.perf_counter() - t0
     cells = sum(c.Nx * c.Ny for c in cases)
     gbs = 72.0 * cells * steps / (dev_ms * 1e-3) / 1e9
-    log(f"batched {n} lattices x {steps} steps FP32: {wall:.2f} s wall, {dev_ms / 1e3:.2f} s "
+    log(f"batched {n} lattices x {steps} steps (precision {prec}): {wall:.2f} s wall, "
+        f"{dev_ms / 1e3:.2f} s "
         f"device, {n / wall:.1f} solves/s, {cells * steps / wall / 1e6:.0f} MLUPS, "
         f"{gbs:.0f} GB/s, tuning {tuning} ({t_tune:.1f} s)")
 
@@ -143,11 +149,14 @@ def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True):
     seq_total = seq * n / len(pick)
     worst = max(max(v) for v in err_char.values())
     log(f"sequential FP64 (sample {len(pick)}): {seq:.2f} s -> {seq_total:.1f} s for {n}, "
-        f"{n / seq_total:.2f} solves/s; FP32 velocity error (RMS / U_char) max {worst:.2e}; "
+        f"{n / seq_total:.2f} solves/s; velocity error vs FP64 (RMS / U_char) max {worst:.2e}; "
         + ", ".join(f"{k}: {max(v):.1e} (field-relative {max(err_field[k]):.1e})"
                     for k, v in sorted(err_char.items())))
     return {
-        "workload": f"batched_{n}_lattices_{steps}_steps_fp32_shifted",
+        "workload": f"batched_{n}_lattices_{steps}_steps_"
+                    + {native.F32_WIDE: "fp32_storage_fp64_collision",
+                       native.F32_SHIFTED: "fp32_storage_fp32_collision",
+                       native.F64: "fp64"}[prec],
         "solves_per_s": round(n / wall, 2), "wall_s": round(wall, 3),
         "device_s": round(dev_ms / 1e3, 3), "MLUPS": round(cells * steps / wall / 1e6, 1),
         "GB_per_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
@@ -156,8 +165,8 @@ def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True):
         "sequential_fp64_solves_per_s": round(n / seq_total, 3),
         "sequential_sample": len(pick),
         "speedup_vs_sequential": round((n / wall) / (n / seq_total), 2),
-        "fp32_velocity_err_rms_over_uchar_max": float(f"{worst:.3e}"),
-        "fp32_velocity_err_by_family": {
+        "velocity_err_vs_fp64_rms_over_uchar_max": float(f"{worst:.3e}"),
+        "velocity_err_vs_fp64_by_family": {
             k: {"rms_over_uchar_max": float(f"{max(v):.3e}"),
                 "field_relative_l2_max": float(f"{max(err_field[k]):.3e}"), "n": len(v)}
             for k, v in sorted(err_char.items())},
@@ -341,7 +350,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
-              "decomposition is described in DESIGN.md and not built", file=sys.stderr)
+              "slab decomposition is a design, not built", file=sys.stderr)
         sys.exit(2)
 
     from lbc_amd import native
@@ -365,10 +374,21 @@ def main():
         lat.close()
         m32 = cells * args.steps / w32 / 1e6
         secondary["fp32_shifted_tgv"] = {
-            "workload": f"taylor_green_periodic_{N}x{N}_fp32_shifted", "MLUPS": round(m32, 1),
+            "workload": f"taylor_green_periodic_{N}x{N}_fp32_storage_fp32_collision",
+            "MLUPS": round(m32, 1),
             "GB_per_s": round(72.0 * cells / (d32 / args.steps * 1e-3) / 1e9, 1),
             "vs_fp64": round(m32 / mlups, 3)}
         log(f"FP32-shifted {N}^2: {m32:.0f} MLUPS ({m32 / mlups:.2f}x FP64)")
+        lat = tgv_lattice(N, native.F32_WIDE)
+        ww, dw = time_lattice(lat, args.steps, args.warmup)
+        lat.close()
+        mw = cells * args.steps / ww / 1e6
+        secondary["fp32_wide_tgv"] = {
+            "workload": f"taylor_green_periodic_{N}x{N}_fp32_storage_fp64_collision",
+            "MLUPS": round(mw, 1),
+            "GB_per_s": round(72.0 * cells / (dw / args.steps * 1e-3) / 1e9, 1),
+            "vs_fp64": round(mw / mlups, 3)}
+        log(f"FP32 storage, FP64 collision {N}^2: {mw:.0f} MLUPS ({mw / mlups:.2f}x FP64)")
         lat = cylinder_lattice(native.F64)
         wc, dc = time_lattice(lat, 500, 50)
         lat.close()

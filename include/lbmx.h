@@ -175,6 +175,14 @@ LBMX_API int lbmx_time_steps(lbmx_lattice *lat, int nsteps, double *ms);
 
 /* Momentum-exchange force of the last step, as main_lbm returns it through
  * (L, D) (main_lbm.c:43): D = x-component, L = y-component. */
+/* Force record on the device: from the next step on, every step appends the row
+ * {t = row*dt, L*cf, D*cf} -- the record the cylinder driver writes per step
+ * (cases/cylinder/cylinder.c:162-185) -- without a host read per step.
+ * capacity 0 turns it off. */
+LBMX_API int lbmx_set_force_record(lbmx_lattice *lat, double dt, double cf, int capacity);
+/* Copies up to capacity recorded rows (3 doubles each); returns the row count. */
+LBMX_API int lbmx_force_record(lbmx_lattice *lat, double *rows, int capacity);
+
 LBMX_API int lbmx_forces(lbmx_lattice *lat, double *L, double *D);
 
 /* Moments of the current state exactly as compute_macroscopic_fields leaves

@@ -25,8 +25,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
 
 # -ffp-contract=off: FP64 kernels keep the reference's separate multiply and add
-# roundings (its x86-64 build has no FMA); FP32 code opts back in locally with
-# `#pragma clang fp contract(fast)`.  Packed-FP32 instructions are disabled.
+# roundings (its x86-64 build has no FMA); the FP32 collision writes its fused
+# multiply-adds explicitly (__fmaf_rn).  Packed-FP32 instructions are disabled.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fvisibility=hidden",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -205,6 +205,7 @@ extern "C" void lbmx_destroy(lbmx_lattice *L)
     hipFree(L->links);
     hipFree(L->partials);
     hipFree(L->force_dev);
+    if (L->frec) hipFree(L->frec);
     hipFree(L->bc_dev);
     for (double *p : L->val1) hipFree(p);
     for (double *p : L->val2) hipFree(p);
@@ -730,7 +731,7 @@ static int launch_step(lbmx_lattice *L)
                            (const T *)S, D, la);
         HIPCHK(hipGetLastError());
     }
-    if (L->nprog || L->nlinks) {
+    if (L->nprog || L->nlinks || L->frec) {
         BcArgs bc;
         bc.g = g;
         bc.s = s;
@@ -740,6 +741,11 @@ static int launch_step(lbmx_lattice *L)
         bc.partials = L->partials;
         bc.npartials = L->nlinks ? L->npartials : 0;
         bc.force_out = L->force_dev;
+        bc.record = L->frec;
+        bc.row = L->frec_row++;
+        bc.capacity = L->frec_cap;
+        bc.dt = L->frec_dt;
+        bc.cf = L->frec_cf;
         hipLaunchKernelGGL(k_bc<T>, dim3(1), dim3(kBcThreads), 0, L->stream, (const T *)S, D,
                            bc);
         HIPCHK(hipGetLastError());
@@ -773,7 +779,7 @@ static bool pull_eligible(const lbmx_lattice *L)
     static int disabled = -1;
     if (disabled < 0) disabled = getenv("LBMX_NO_GATHER") ? 1 : 0;
     return !disabled && L->geom == 0 && L->cfg.per_x && L->cfg.per_y && L->nprog == 0 &&
-           !L->tdibc.on && L->probes.n == 0 && !(L->cfg.flags & LBMX_EXACT_STATE);
+           !L->tdibc.on && L->probes.n == 0 && !L->frec && !(L->cfg.flags & LBMX_EXACT_STATE);
 }
 
 static PullArgs pull_args(const lbmx_lattice *L)
@@ -1082,6 +1088,37 @@ extern "C" int lbmx_forces(lbmx_lattice *L, double *Lift, double *Drag)
     if (Drag) *Drag = L->force_host[0];
     if (Lift) *Lift = L->force_host[1];
     return LBMX_OK;
+}
+
+extern "C" int lbmx_set_force_record(lbmx_lattice *L, double dt, double cf, int capacity)
+{
+    if (!L || capacity < 0) return lbmx_fail(LBMX_ERR_ARG, "invalid argument");
+    if (int prc = lbmx_ensure_push(L)) return prc;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    if (L->frec) hipFree(L->frec);
+    L->frec = nullptr;
+    L->frec_cap = 0;
+    L->frec_row = 0;
+    if (capacity == 0) return LBMX_OK;
+    if (hipMalloc((void **)&L->frec, (size_t)capacity * 3 * sizeof(double)) != hipSuccess) {
+        L->frec = nullptr;
+        return lbmx_fail(LBMX_ERR_NOMEM, "force record of %d rows", capacity);
+    }
+    L->frec_cap = capacity;
+    L->frec_dt = dt;
+    L->frec_cf = cf;
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_force_record(lbmx_lattice *L, double *rows, int capacity)
+{
+    if (!L || !rows || capacity < 0) return lbmx_fail(LBMX_ERR_ARG, "invalid argument");
+    if (!L->frec) return 0;
+    long long n = L->frec_row < L->frec_cap ? L->frec_row : L->frec_cap;
+    if (n > capacity) n = capacity;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    if (n > 0) HIPCHK(hipMemcpy(rows, L->frec, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return (int)n;
 }
 
 int lbmx_launch_moments(lbmx_lattice *L)

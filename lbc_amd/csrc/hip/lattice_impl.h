@@ -48,6 +48,11 @@ struct lbmx_lattice {
     int npartials = 0;
     double *force_dev = nullptr;
     double *force_host = nullptr;
+    // force record {t, L, D} per step (lbmx_set_force_record)
+    double *frec = nullptr;
+    int frec_cap = 0;
+    long long frec_row = 0;
+    double frec_dt = 1.0, frec_cf = 1.0;
     int edge_rows = 0, edge_cols = 0;
 
     // boundary table (as given) and the device program compiled from it

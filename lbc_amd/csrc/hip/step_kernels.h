@@ -537,6 +537,10 @@ struct BcArgs {
     const double *partials;
     int npartials;
     double *force_out;   // [0] = x (the reference's D), [1] = y (L)
+    double *record;      // optional {t, L, D} rows (cases/cylinder/cylinder.c:162-185)
+    long long row;       // row of this step
+    int capacity;
+    double dt, cf;       // t = row * dt, L and D scaled by cf
 };
 
 template <typename T>
@@ -783,6 +787,13 @@ __global__ void __launch_bounds__(kBcThreads) k_bc(const T *__restrict__ S, T *_
         }
         a.force_out[0] = fx;
         a.force_out[1] = fy;
+    }
+    if (a.record && threadIdx.x == 0 && a.row < a.capacity) {
+        const double fx = a.npartials > 0 ? a.force_out[0] : 0.0;
+        const double fy = a.npartials > 0 ? a.force_out[1] : 0.0;
+        a.record[3 * a.row + 0] = (double)a.row * a.dt;
+        a.record[3 * a.row + 1] = fy * a.cf;
+        a.record[3 * a.row + 2] = fx * a.cf;
     }
 }
 

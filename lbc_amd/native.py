@@ -92,6 +92,8 @@ def _bind():
     L.lbmx_mei_initialization.argtypes = [i, i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                           i, i, i, d, _ip]
     L.lbmx_forces.argtypes = [vp, _dp, _dp]
+    L.lbmx_set_force_record.argtypes = [vp, d, d, i]
+    L.lbmx_force_record.argtypes = [vp, _dp, i]
     L.lbmx_moments.argtypes = [vp, _dp, _dp, _dp]
     L.lbmx_download.argtypes = [vp, _dp, _dp]
     L.lbmx_synchronize.argtypes = [vp]
@@ -235,6 +237,17 @@ class Lattice:
 
     def step(self, n=1):
         check(self._L.lbmx_step(self.h, int(n)))
+
+    def set_force_record(self, capacity, dt=1.0, cf=1.0):
+        check(self._L.lbmx_set_force_record(self.h, float(dt), float(cf), int(capacity)))
+        self._frec_cap = int(capacity)
+
+    def force_record(self):
+        rows = np.zeros((max(1, self._frec_cap), 3))
+        n = self._L.lbmx_force_record(self.h, _d(rows), rows.shape[0])
+        if n < 0:
+            check(n)
+        return rows[:n]
 
     def forces(self):
         Lf, Df = C.c_double(), C.c_double()

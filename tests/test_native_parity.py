@@ -3,9 +3,9 @@
 Every scenario of the battery (all boundary types, bounce-back and interpolated
 bounce-back, Guo forcing, periodic and non-periodic edges, stale entries,
 corners, boundaries at interior indices) is run for several steps from the same
-seeded state on the oracle and on the MI355X.  FP64 must agree to the
-north-star tolerance (1e-12 relative in rho and u); in practice the HIP path is
-bit-identical and the tests also report that.  FP32-shifted storage is checked
+seeded state on the oracle and on the MI355X.  FP64 must agree to 1e-12
+relative in rho and u; in practice the HIP path is bit-identical and the tests
+also assert that.  FP32-shifted storage is checked
 against the FP64 oracle with a stated tolerance.
 """
 import numpy as np
@@ -159,3 +159,32 @@ def test_fp32_shifted_close_to_fp64(sc, precision):
     nu = np.sqrt(np.sum(ref.u[fluid] ** 2 + ref.v[fluid] ** 2))
     assert du / nu <= FP32_VEL_L2[precision], du / nu
     assert rel_err(got["rho"], ref.rho, fluid) <= 1e-6
+
+
+FORCE_CASES = [s for s in SCENARIOS if s.use_ibb or s.mask.any()][:4]
+
+
+@pytest.mark.parametrize("sc", FORCE_CASES, ids=[f"{s.name}_{s.Nx}x{s.Ny}" for s in FORCE_CASES])
+def test_force_record_matches_oracle(sc):
+    """The device force record (cylinder.c:162-185 layout {t, L*cf, D*cf}) equals
+    the oracle's per-step (L, D), up to the fixed summation order."""
+    from lbc_amd import native
+    state = sc.initial_state()
+    _, ref_forces = run_oracle(sc, state, NSTEPS)
+    lat = native.Lattice(sc.Nx, sc.Ny, precision=0, per_x=sc.per_x, per_y=sc.per_y,
+                         use_ibb=sc.use_ibb, Fx=sc.Fx, Fy=sc.Fy)
+    lat.set_geometry(sc.mask, sc.omega, sc.phi)
+    lat.set_boundaries(sc.bc_names)
+    lat.upload(*state)
+    dt, cf = 1.0 / 300.0, 2.5
+    lat.set_force_record(NSTEPS, dt=dt, cf=cf)
+    for step in range(NSTEPS):
+        for b, (_, _, v1, v2) in enumerate(sc.vals(step)):
+            lat.set_bc_values(b, v1, v2)
+        lat.step(1)
+    rec = lat.force_record()
+    lat.close()
+    assert rec.shape == (NSTEPS, 3)
+    assert np.array_equal(rec[:, 0], np.arange(NSTEPS) * dt)
+    np.testing.assert_allclose(rec[:, 1], ref_forces[:, 0] * cf, rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(rec[:, 2], ref_forces[:, 1] * cf, rtol=1e-11, atol=1e-12)

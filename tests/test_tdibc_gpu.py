@@ -8,7 +8,8 @@
   step 1900): bit-identical to the oracle's stored rows.
 * The full uniform run (25,047 steps) tracks the reference's published
   tdibc_2000Hz_89Pa.txt series (physics-level agreement; that file comes from a
-  nit version not in the tree).
+  nit version not in the tree: the tree's nit.c, built from the reference's
+  own sources, diverges from it within ~60 steps).
 * Liner duct (per-node TDIBC on a regularized wall, sponge ends): FP64 equals
   the oracle bit for bit; FP32-shifted against FP64 within a stated tolerance.
 """
