@@ -749,12 +749,15 @@ LBMX_API int lbmx_batch_tune(lbmx_batch *b, int trial_steps, lbmx_tuning *chosen
     double best = 1e300;
     lbmx_tuning pick = keep;
     int rc = LBMX_OK;
+    // initial states once (trial epochs start at 0); each candidate then only gets
+    // its own tile table and the epoch back at 0, so its steps stay in range
+    b->epoch = 0;
+    b->dirty = true;
     for (const lbmx_tuning &t : cand) {
         b->tun = t;
         b->dirty = true;
         b->epoch = 0;
-        for (int slot : b->order) b->slots[slot].pending = 1;
-        rc = commit(b);   // initial states on scratch epochs, trial table
+        rc = commit(b);
         if (!rc) rc = lbmx_batch_enqueue(b, 2);
         double ms = 0.0;
         if (!rc) rc = lbmx_batch_time(b, trial_steps, &ms);
