@@ -254,6 +254,85 @@ def serving_bench(rates=(10, 30, 100, 300, 1000), n=32, seq_sample=6, seed=2026,
     }
 
 
+def cylinder_bench(steps=50000):
+    """Channel past a cylinder, 2048 x 512, Re = 100: FP64 and FP32 storage with the
+    FP64 collision, `steps` steps each with the drag/lift history recorded on the
+    device every step (the cylinder driver's {t, L, D} record).  Reported: MLUPS
+    and the FP32 history's largest deviation from FP64, relative to the FP64
+    run's largest |D| and |L|."""
+    from lbc_amd import native
+    out = {"workload": f"cylinder_channel_2048x512_re100_ibb_regularized_{steps}_steps"}
+    rec = {}
+    for name, prec in (("fp64", native.F64), ("fp32_fp64collision", native.F32_WIDE)):
+        lat = cylinder_lattice(prec)
+        lat.set_force_record(steps)
+        lat.synchronize()
+        t0 = time.perf_counter()
+        lat.step(steps)
+        lat.synchronize()
+        wall = time.perf_counter() - t0
+        rec[name] = lat.force_record()
+        lat.close()
+        out[name] = {"MLUPS": round(2048 * 512 * steps / wall / 1e6, 1),
+                     "ms_per_step": round(wall / steps * 1e3, 4)}
+    r64, r32 = rec["fp64"], rec["fp32_fp64collision"]
+    n = min(len(r64), len(r32))
+    out["fp32_fp64collision"]["drag_err"] = float(
+        f"{np.abs(r32[:n, 2] - r64[:n, 2]).max() / np.abs(r64[:n, 2]).max():.3e}")
+    out["fp32_fp64collision"]["lift_err"] = float(
+        f"{np.abs(r32[:n, 1] - r64[:n, 1]).max() / np.abs(r64[:n, 1]).max():.3e}")
+    out["fp64"]["drag_last"] = float(r64[n - 1, 2])
+    out["fp64"]["lift_last"] = float(r64[n - 1, 1])
+    out["vs_fp64"] = round(out["fp32_fp64collision"]["MLUPS"] / out["fp64"]["MLUPS"], 3)
+    log(f"cylinder: {out}")
+    return out
+
+
+def liner_bench(steps=20000):
+    """The acoustic liner duct, 8192 x 256, tau = 0.51: Gaussian pulse (1e-3 rho0,
+    half-width 20, x = 1024) at rest, rigid top wall, impedance wall (mass-spring-
+    damper, R = rho0 c0, resonance 1/40 per step, M = 8) on x in [3072, 5120] of the
+    bottom wall, convective ends with sponges; 20,000 steps in FP64 (bit-identical
+    to the oracle) and in both FP32-storage modes.  Reported: time per run, and the
+    FP32 modes' largest error, relative to the pulse's p' amplitude, in the wall
+    pressure (three wall nodes over time and the whole wall at the end) and in the
+    transmitted pressure at x = 6144."""
+    from lbc_amd import batch as B
+    from lbc_amd import cases as CS_
+    from lbc_amd import native
+    imp = CS_.LinerSpec(R=1.0, f_res=1.0 / 40.0, M=8.0).impedance()
+    probes = [(3584, 0), (4096, 0), (4608, 0), (6144, 128)]
+    out = {"workload": f"liner_duct_8192x256_tau0.51_{steps}_steps"}
+    runs = {}
+    for name, prec in (("fp64", native.F64), ("fp32_fp32collision", native.F32_SHIFTED),
+                       ("fp32_fp64collision", native.F32_WIDE)):
+        c = B.liner_duct(8192, 256, steps, imp, tau=0.51, amp=1e-3, x0=1024.0,
+                         halfwidth=20.0, liner=(3072, 5120), sponge=192, probes=probes)
+        b = B.Batch(prec, budget=B.case_bytes(c, prec) + (64 << 20))
+        i = b.add(c)
+        b.synchronize()
+        ms = b.time(steps)
+        hist = b.probes(i)
+        rho, _, _ = b.moments(i)
+        b.close()
+        runs[name] = (hist, rho)
+        out[name] = {"s": round(ms / 1e3, 3), "MLUPS": round(8192 * 256 * steps / ms / 1e3, 1)}
+    cs2 = 1.0 / 3.0
+    scale = 1e-3 * cs2                       # the pulse's initial p' amplitude
+    h64, r64 = runs["fp64"]
+    wall64 = cs2 * (r64[0, 3072:5121] - 1.0)
+    for name in ("fp32_fp32collision", "fp32_fp64collision"):
+        h, r = runs[name]
+        out[name]["wall_p_series_err"] = float(f"{cs2 * np.abs(h[:, :3] - h64[:, :3]).max() / scale:.3e}")
+        out[name]["wall_p_final_err"] = float(
+            f"{np.abs(cs2 * (r[0, 3072:5121] - 1.0) - wall64).max() / scale:.3e}")
+        out[name]["transmitted_err"] = float(f"{cs2 * np.abs(h[:, 3] - h64[:, 3]).max() / scale:.3e}")
+    out["transmitted_peak_over_incident"] = float(
+        f"{cs2 * np.abs(h64[:, 3] - 1.0).max() / (scale / 2):.4f}")
+    log(f"liner duct: {out}")
+    return out
+
+
 def characteristic_velocity(c):
     """Velocity scale of a batched case: the wall speed (Couette, cavity), the
     Taylor-Green amplitude, or for the force-driven channel the smaller of the
@@ -333,6 +412,9 @@ def main():
                     help="lattices timed one by one for the sequential FP64 baseline")
     ap.add_argument("--serving", type=int, default=32, metavar="N",
                     help="requests per arrival rate of the serving sweep (0: skip)")
+    ap.add_argument("--cylinder-steps", type=int, default=50000)
+    ap.add_argument("--liner", type=int, default=20000, metavar="STEPS",
+                    help="steps of the 8192 x 256 liner duct in FP64 and FP32 (0: skip)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run one plain device-to-device copy of GIB GiB (torch) and "
                          "exit: the known-size reference for scaling rocprofv3 "
@@ -389,19 +471,14 @@ def main():
             "GB_per_s": round(72.0 * cells / (dw / args.steps * 1e-3) / 1e9, 1),
             "vs_fp64": round(mw / mlups, 3)}
         log(f"FP32 storage, FP64 collision {N}^2: {mw:.0f} MLUPS ({mw / mlups:.2f}x FP64)")
-        lat = cylinder_lattice(native.F64)
-        wc, dc = time_lattice(lat, 500, 50)
-        lat.close()
-        secondary["cylinder_fp64"] = {
-            "workload": "cylinder_channel_2048x512_fp64_ibb_regularized",
-            "MLUPS": round(2048 * 512 * 500 / wc / 1e6, 1),
-            "ms_per_step": round(wc / 500 * 1e3, 4)}
-        log(f"cylinder 2048x512 FP64: {secondary['cylinder_fp64']}")
+        secondary["cylinder"] = cylinder_bench(args.cylinder_steps)
     if args.batched > 0:
         secondary["batched"] = batched_bench(args.batched, args.batched_steps,
                                              args.batched_sample)
     if args.serving > 0:
         secondary["serving"] = serving_bench(n=args.serving)
+    if args.liner > 0:
+        secondary["liner_duct"] = liner_bench(args.liner)
 
     cpu = None
     if not args.no_cpu:
