@@ -415,6 +415,7 @@ extern "C" int lbmx_set_boundaries(lbmx_lattice *L, const int *types, const int 
     // Leading periodic entries whose axis already wraps in streaming re-write
     // exactly what streaming wrote (periodic_x.c vs streaming.c:24-34): dropped.
     std::vector<BcDesc> prog;
+    std::vector<int> edge;
     bool only_redundant_so_far = true;
     for (int b = 0; b < nbc; b++) {
         const int t = types[b];
@@ -432,7 +433,9 @@ extern "C" int lbmx_set_boundaries(lbmx_lattice *L, const int *types, const int 
         d.v1 = L->val1[b];
         d.v2 = L->val2[b];
         prog.push_back(d);
+        edge.push_back((lbmx_bc_on_column(t) || t == LBMX_BC_PERIODIC_X) ? L->Ny : L->Nx);
     }
+    L->prog_edge = edge;
     hipFree(L->bc_dev);
     L->bc_dev = nullptr;
     L->nprog = (int)prog.size();
@@ -746,8 +749,15 @@ static int launch_step(lbmx_lattice *L)
         bc.capacity = L->frec_cap;
         bc.dt = L->frec_dt;
         bc.cf = L->frec_cf;
-        hipLaunchKernelGGL(k_bc<T>, dim3(1), dim3(kBcThreads), 0, L->stream, (const T *)S, D,
-                           bc);
+        if (L->nprog == 0) {
+            hipLaunchKernelGGL(k_bc_one<T>, dim3(1), dim3(kBcOneThreads), 0, L->stream,
+                               (const T *)S, D, bc, 0, 1);
+        }
+        for (int b = 0; b < L->nprog; b++) {
+            const int blocks = (L->prog_edge[b] + kBcOneThreads - 1) / kBcOneThreads;
+            hipLaunchKernelGGL(k_bc_one<T>, dim3(blocks), dim3(kBcOneThreads), 0, L->stream,
+                               (const T *)S, D, bc, b, b == 0 ? 1 : 0);
+        }
         HIPCHK(hipGetLastError());
     }
     if (!s.exact && (L->edge_rows + L->edge_cols) > 0) {

@@ -60,6 +60,7 @@ struct lbmx_lattice {
     std::vector<double *> val1, val2;
     lbmx::BcDesc *bc_dev = nullptr;
     int nprog = 0;
+    std::vector<int> prog_edge;   // edge length of each program entry
 
     int has_geometry = 0, has_state = 0, lag_arrays = 0;
     double Fx_lag = 0.0, Fy_lag = 0.0;

@@ -543,6 +543,11 @@ struct BcArgs {
     double dt, cf;       // t = row * dt, L and D scaled by cf
 };
 
+// Edge nodes of one boundary are independent of each other: the threads of every
+// workgroup of the launch share them (one workgroup in k_bc, many in k_bc_one).
+__device__ __forceinline__ int bc_first() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int bc_stride() { return gridDim.x * blockDim.x; }
+
 template <typename T>
 __device__ void bc_zou_he_column(const T *S, T *D, const BcArgs &a, const BcDesc &d)
 {
@@ -553,7 +558,7 @@ __device__ void bc_zou_he_column(const T *S, T *D, const BcArgs &a, const BcDesc
     const int i = d.index;
     const bool inlet = (d.type == 7 || d.type == 3);
     const bool pressure = (d.type == 3 || d.type == 4);
-    for (int j = threadIdx.x; j < g.Ny; j += blockDim.x) {
+    for (int j = bc_first(); j < g.Ny; j += bc_stride()) {
         const size_t n = (size_t)j * g.pitch + i;
         if (solid_at(g, n)) continue;
         double f[Q];
@@ -650,7 +655,7 @@ __device__ void bc_zou_he_row(T *D, const BcArgs &a, const BcDesc &d)
     const size_t P = g.plane;
     const int j = d.index;
     const bool top = (d.type == 9);
-    for (int i = threadIdx.x; i < g.Nx; i += blockDim.x) {
+    for (int i = bc_first(); i < g.Nx; i += bc_stride()) {
         const size_t n = (size_t)j * g.pitch + i;
         if (solid_at(g, n)) continue;
         const double U = d.v1[i], V = d.v2[i];
@@ -685,7 +690,7 @@ __device__ void bc_regularized(const T *S, T *D, const BcArgs &a, const BcDesc &
     const bool pressure = (t >= 15 && t <= 18);
     const int step_in = (t == 12 || t == 14 || t == 16 || t == 18) ? 1 : -1;
     const int nedge = column ? g.Ny : g.Nx;
-    for (int e = threadIdx.x; e < nedge; e += blockDim.x) {
+    for (int e = bc_first(); e < nedge; e += bc_stride()) {
         const int ib = column ? d.index : e, jb = column ? e : d.index;
         const int ii = column ? d.index + step_in : e, ji = column ? e : d.index + step_in;
         const size_t nb = (size_t)jb * g.pitch + ib, ni = (size_t)ji * g.pitch + ii;
@@ -721,7 +726,7 @@ __device__ void bc_convective(T *D, const BcArgs &a, const BcDesc &d)
     const bool column = (t == 1 || t == 2);
     const int from = d.index + ((t == 1 || t == 20) ? 1 : -1);
     const int nedge = column ? g.Ny : g.Nx;
-    for (int e = threadIdx.x; e < nedge; e += blockDim.x) {
+    for (int e = bc_first(); e < nedge; e += bc_stride()) {
         const size_t nb = column ? (size_t)e * g.pitch + d.index : (size_t)d.index * g.pitch + e;
         const size_t ns = column ? (size_t)e * g.pitch + from : (size_t)from * g.pitch + e;
         if (solid_at(g, nb)) continue;
@@ -739,7 +744,7 @@ __device__ void bc_periodic(const T *S, T *D, const BcArgs &a, bool along_x)
     const size_t P = g.plane;
     const int nedge = along_x ? g.Ny : g.Nx;
     const int L = along_x ? g.Nx : g.Ny;
-    for (int e = threadIdx.x; e < nedge; e += blockDim.x) {
+    for (int e = bc_first(); e < nedge; e += bc_stride()) {
 #pragma unroll
         for (int k = 1; k < Q; k++) {
             const int c = along_x ? D2Q9::cx[k] : D2Q9::cy[k];
@@ -764,22 +769,22 @@ __device__ void bc_periodic(const T *S, T *D, const BcArgs &a, bool along_x)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBcThreads) k_bc(const T *__restrict__ S, T *__restrict__ D,
-                                                   BcArgs a)
+__device__ __forceinline__ void bc_apply(const T *S, T *D, const BcArgs &a, const BcDesc &d)
 {
-    for (int b = 0; b < a.nbc; b++) {
-        const BcDesc d = a.bcs[b];
-        switch (d.type) {
-        case 3: case 4: case 7: case 8: bc_zou_he_column(S, D, a, d); break;
-        case 9: case 10: bc_zou_he_row(D, a, d); break;
-        case 1: case 2: case 19: case 20: bc_convective(D, a, d); break;
-        case 5: bc_periodic(S, D, a, true); break;
-        case 6: bc_periodic(S, D, a, false); break;
-        default: bc_regularized(S, D, a, d); break;
-        }
-        __syncthreads();
+    switch (d.type) {
+    case 3: case 4: case 7: case 8: bc_zou_he_column(S, D, a, d); break;
+    case 9: case 10: bc_zou_he_row(D, a, d); break;
+    case 1: case 2: case 19: case 20: bc_convective(D, a, d); break;
+    case 5: bc_periodic(S, D, a, true); break;
+    case 6: bc_periodic(S, D, a, false); break;
+    default: bc_regularized(S, D, a, d); break;
     }
-    if (a.npartials > 0 && threadIdx.x == 0) {
+}
+
+// main_lbm's force outputs (fixed-order sum of the link partials) and the record
+__device__ __forceinline__ void bc_force_total(const BcArgs &a)
+{
+    if (a.npartials > 0) {
         double fx = 0.0, fy = 0.0;
         for (int p = 0; p < a.npartials; p++) {
             fx += a.partials[2 * p];
@@ -788,13 +793,40 @@ __global__ void __launch_bounds__(kBcThreads) k_bc(const T *__restrict__ S, T *_
         a.force_out[0] = fx;
         a.force_out[1] = fy;
     }
-    if (a.record && threadIdx.x == 0 && a.row < a.capacity) {
+    if (a.record && a.row < a.capacity) {
         const double fx = a.npartials > 0 ? a.force_out[0] : 0.0;
         const double fy = a.npartials > 0 ? a.force_out[1] : 0.0;
         a.record[3 * a.row + 0] = (double)a.row * a.dt;
         a.record[3 * a.row + 1] = fy * a.cf;
         a.record[3 * a.row + 2] = fx * a.cf;
     }
+}
+
+// The whole table in one workgroup (stand-alone stages).
+template <typename T>
+__global__ void __launch_bounds__(kBcThreads) k_bc(const T *__restrict__ S, T *__restrict__ D,
+                                                   BcArgs a)
+{
+    for (int b = 0; b < a.nbc; b++) {
+        bc_apply(S, D, a, a.bcs[b]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bc_force_total(a);
+}
+
+// Table entry b over as many workgroups as its edge needs; the entries run as
+// successive launches on one stream, so each sees its predecessors' writes
+// (corners) exactly as in main_lbm.c:49-58.  The first launch also forms the
+// force outputs (the link partials are complete when it starts).
+constexpr int kBcOneThreads = 128;
+
+template <typename T>
+__global__ void __launch_bounds__(kBcOneThreads) k_bc_one(const T *__restrict__ S,
+                                                          T *__restrict__ D, BcArgs a, int b,
+                                                          int with_force)
+{
+    if (b < a.nbc) bc_apply(S, D, a, a.bcs[b]);
+    if (with_force && blockIdx.x == 0 && threadIdx.x == 0) bc_force_total(a);
 }
 
 // ---- k_edges: stale slots of non-periodic domain edges ----------------------------
