@@ -703,7 +703,7 @@ LBMX_API int lbmx_batch_set_tuning(lbmx_batch *b, const lbmx_tuning *t)
         t->groups_per_cu > 32)
         return lbmx_fail(LBMX_ERR_ARG, "invalid tuning (threads %d, tile %d, groups/CU %d)",
                          t->threads, t->tile, t->groups_per_cu);
-    if (t->order < 0 || t->order > 2) return lbmx_fail(LBMX_ERR_ARG, "tile order must be 0, 1 or 2");
+    if (t->order < 0 || t->order > 4) return lbmx_fail(LBMX_ERR_ARG, "tile order must be 0 .. 4");
     b->tun = *t;
     drop_graphs(b);
     b->dirty = true;
@@ -741,7 +741,7 @@ LBMX_API int lbmx_batch_tune(lbmx_batch *b, int trial_steps, lbmx_tuning *chosen
     for (int threads : {128, 256})
         for (int ilp : {1, 2, 4, 8})
             for (int g : {2, 4, 8, 16})
-                for (int order : {0, 1, 2}) {
+                for (int order : {0, 1, 2, 3, 4}) {
                     const int tile = threads * ilp;
                     if (tile < 256) continue;
                     cand.push_back(lbmx_tuning{threads, tile, g, order});

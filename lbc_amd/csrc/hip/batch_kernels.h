@@ -379,19 +379,39 @@ __host__ __device__ __forceinline__ int frame_nodes(int Nx, int Ny)
 constexpr int kFrameTile = 256;
 constexpr int kFrameThreads = 256;
 
-// Tile range of workgroup b out of G for n tiles.  mode 0: interleaved (tile t on
-// workgroup t mod G, so the chip sweeps the table as one front, as a hardware-
-// dispatched grid would); 1: one contiguous band per XCD (blocks b with equal b % 8
-// run on one XCD); 2: one contiguous range per workgroup.
-struct TileRange {
-    int t0, t1, dt;
+// Tiles of workgroup b out of G for n tiles, as the k-th tile of b's sequence.
+// mode 0: interleaved (tile t on workgroup t mod G, so the chip sweeps the table
+// as one front, as a hardware-dispatched grid would); 1: one contiguous band per
+// XCD (blocks b with equal b % 8 run on one XCD); 2: one contiguous range per
+// workgroup; 3 and 4: interleaved bands of 4 / 16 consecutive tiles, each band on
+// one XCD (band c on XCD c % 8, its tiles dealt round-robin to that XCD's
+// workgroups), so a tile's neighbouring rows are mostly read through the same L2
+// while the chip still sweeps the table as one front.
+struct TileSeq {
+    int mode, b, G, n, t0, t1, B;
+    __device__ __forceinline__ int at(int k) const
+    {
+        if (mode == 0) return b + k * G;
+        if (mode == 1 || mode == 2) return t0 + k < t1 ? t0 + k : n;
+        const int G8 = G / 8, x = b % 8, q = b / 8 + k * G8;
+        return (x + 8 * (q / B)) * B + q % B;
+    }
 };
-__device__ __forceinline__ TileRange tile_range(int n, int mode)
+__device__ __forceinline__ TileSeq tile_seq(int n, int mode)
 {
-    const int G = gridDim.x, b = blockIdx.x;
-    if (mode == 0) return TileRange{b, n, G};
-    const int chunk = mode == 1 ? (b % 8) * (G / 8) + b / 8 : b;
-    return TileRange{(int)((long long)n * chunk / G), (int)((long long)n * (chunk + 1) / G), 1};
+    TileSeq s;
+    s.mode = mode;
+    s.b = blockIdx.x;
+    s.G = gridDim.x;
+    s.n = n;
+    s.B = mode == 3 ? 4 : 16;
+    s.t0 = s.t1 = 0;
+    if (mode == 1 || mode == 2) {
+        const int chunk = mode == 1 ? (s.b % 8) * (s.G / 8) + s.b / 8 : s.b;
+        s.t0 = (int)((long long)n * chunk / s.G);
+        s.t1 = (int)((long long)n * (chunk + 1) / s.G);
+    }
+    return s;
 }
 
 // One time step of every lattice whose own step lies in [0, steps): rows 1 .. Ny-2.
@@ -402,8 +422,11 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_interior(
     const BatchCtl *__restrict__ ctl, int local, int tile, int mode)
 {
     const long long epoch = ctl->epoch + local;
-    const TileRange R = tile_range(ctl->n_interior, mode);
-    for (int t = R.t0; t < R.t1; t += R.dt) {
+    const int n = ctl->n_interior;
+    const TileSeq Q = tile_seq(n, mode);
+    for (int k = 0;; k++) {
+        const int t = Q.at(k);   // increasing in k
+        if (t >= n) break;
         const Tile tl = tiles[t];
         const BatchLat &L = lats[tl.lat];
         const long long s = epoch - L.start;
@@ -431,8 +454,11 @@ __global__ void __launch_bounds__(kFrameThreads) k_batch_frame(
 {
     const long long epoch = ctl->epoch + local;
     const int base = ctl->n_interior;
-    const TileRange R = tile_range(ctl->n_frame, 0);
-    for (int t = R.t0; t < R.t1; t += R.dt) {
+    const int n = ctl->n_frame;
+    const TileSeq Q = tile_seq(n, 0);
+    for (int k = 0;; k++) {
+        const int t = Q.at(k);
+        if (t >= n) break;
         const Tile tl = tiles[base + t];
         const BatchLat &L = lats[tl.lat];
         const long long s = epoch - L.start;
