@@ -203,6 +203,17 @@ LBMX_API int lbmx_synchronize(lbmx_lattice *lat);
 LBMX_API int64_t lbmx_steps_done(const lbmx_lattice *lat);
 LBMX_API int64_t lbmx_device_bytes(const lbmx_lattice *lat);
 
+/* How lbmx_step groups the work after streaming into launches (no counterpart in
+ * the reference; host only, no device needed).  For a lattice of this size,
+ * periodicity, solid mask (Ny x Nx, or NULL) and boundary table, launch_of[b]
+ * receives the launch (0, 1, ...) that applies table entry b, or -1 for a leading
+ * periodic entry the step leaves out; *edge_launch the launch of the stale-edge
+ * bookkeeping, or -1 on a fully periodic lattice.  Entries sharing a launch touch
+ * disjoint nodes.  Returns the number of launches or a negative error code. */
+LBMX_API int lbmx_tail_plan(int Nx, int Ny, int per_x, int per_y, const int *mask,
+                            const int *types, const int *index, int nbc, int *launch_of,
+                            int *edge_launch);
+
 /* Mei consistent initialization on the device (mei_initialization.c:7-142):
  * same arguments and in/out contract as the reference function with the
  * lattice tables implied.  f/f_new are read as the starting buffers and

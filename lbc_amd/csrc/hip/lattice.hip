@@ -5,6 +5,7 @@
 // buffer management.  Every per-step operation runs on the device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -272,6 +273,12 @@ extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double 
     hipFree(L->lut); L->lut = nullptr;
     hipFree(L->om_arr); L->om_arr = nullptr;
     L->nsolid = (long long)nsolid;
+    L->hsolid.clear();
+    if (nsolid) {
+        L->hsolid.resize(nn);
+        for (size_t n = 0; n < nn; n++) L->hsolid[n] = mask[n] ? 1 : 0;
+    }
+    L->tail_planned = 0;
     if (nsolid == 0 && cls_of.size() == 1) {
         L->geom = 0;
         L->om_uniform = lut[1];
@@ -381,6 +388,8 @@ extern "C" int lbmx_set_uniform(lbmx_lattice *L, double omega)
     L->geom = 0;
     L->om_uniform = omega;
     L->nsolid = 0;
+    L->hsolid.clear();
+    L->tail_planned = 0;
     L->nlinks = 0;
     L->npartials = 0;
     HIPCHK(hipMemsetAsync(L->force_dev, 0, 2 * sizeof(double), L->stream));
@@ -394,6 +403,8 @@ extern "C" int lbmx_set_uniform(lbmx_lattice *L, double omega)
 // ---------------------------------------------------------------------------
 // boundary table
 // ---------------------------------------------------------------------------
+static std::vector<int> program_entries(const int *types, int nbc, int per_x, int per_y);
+
 extern "C" int lbmx_set_boundaries(lbmx_lattice *L, const int *types, const int *index, int nbc)
 {
     if (!L || nbc < 0 || (nbc > 0 && (!types || !index)))
@@ -412,27 +423,28 @@ extern "C" int lbmx_set_boundaries(lbmx_lattice *L, const int *types, const int 
     L->index.assign(index, index + nbc);
     L->vals_set.assign(nbc, 0);
 
-    // Leading periodic entries whose axis already wraps in streaming re-write
-    // exactly what streaming wrote (periodic_x.c vs streaming.c:24-34): dropped.
     std::vector<BcDesc> prog;
     std::vector<int> edge;
-    bool only_redundant_so_far = true;
+    L->prog_type.clear();
+    L->prog_index.clear();
+    L->tail_planned = 0;
     for (int b = 0; b < nbc; b++) {
         const int t = types[b];
-        const bool redundant = (t == LBMX_BC_PERIODIC_X && L->cfg.per_x) ||
-                               (t == LBMX_BC_PERIODIC_Y && L->cfg.per_y);
         const int nv = lbmx_bc_nvals(t);
         const int len = lbmx_bc_on_column(t) ? L->Ny : L->Nx;
         if (nv >= 1) HIPCHK(hipMalloc((void **)&L->val1[b], len * sizeof(double)));
         if (nv >= 2) HIPCHK(hipMalloc((void **)&L->val2[b], len * sizeof(double)));
-        if (redundant && only_redundant_so_far) continue;
-        only_redundant_so_far = false;
+    }
+    for (int b : program_entries(types, nbc, L->cfg.per_x, L->cfg.per_y)) {
+        const int t = types[b];
         BcDesc d;
         d.type = t;
         d.index = index[b];
         d.v1 = L->val1[b];
         d.v2 = L->val2[b];
         prog.push_back(d);
+        L->prog_type.push_back(t);
+        L->prog_index.push_back(index[b]);
         edge.push_back((lbmx_bc_on_column(t) || t == LBMX_BC_PERIODIC_X) ? L->Ny : L->Nx);
     }
     L->prog_edge = edge;
@@ -683,6 +695,234 @@ template <typename T>
 static int launch_probes(lbmx_lattice *L, const T *S, const Geo &g, const StepFlags &s,
                          const Fields &F);
 
+// ---------------------------------------------------------------------------
+// planning the work after streaming (k_tail)
+// ---------------------------------------------------------------------------
+// Nodes (y * Nx + x) a program entry writes in D, reads in D and reads in S or in
+// the lagged-moment arrays, visited as the bc_* kernels visit them.
+namespace {
+struct NodeSets {
+    std::vector<long long> wd, rd, rs;
+};
+
+struct PlanGrid {
+    int Nx, Ny, per_x, per_y;
+    const uint8_t *solid;   // Ny x Nx, or null: no solid node
+};
+
+bool solid_h(const PlanGrid *L, int x, int y)
+{
+    return L->solid && L->solid[(size_t)y * L->Nx + x];
+}
+
+NodeSets entry_nodes(const PlanGrid *L, int t, int idx)
+{
+    static const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    static const int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    const int Nx = L->Nx, Ny = L->Ny;
+    auto id = [Nx](int x, int y) { return (long long)y * Nx + x; };
+    NodeSets s;
+    switch (t) {
+    case 3: case 4: case 7: case 8: {   // Zou-He columns: bc_zou_he_column
+        const bool inlet = (t == 7 || t == 3), pressure = (t == 3 || t == 4);
+        for (int j = 0; j < Ny; j++) {
+            if (solid_h(L, idx, j)) continue;
+            s.wd.push_back(id(idx, j));
+            s.rd.push_back(id(idx, j));
+            if (pressure) s.rs.push_back(id(idx + (inlet ? 1 : -1), j));
+        }
+        break;
+    }
+    case 9: case 10:                    // Zou-He rows: bc_zou_he_row
+        for (int i = 0; i < Nx; i++) {
+            if (solid_h(L, i, idx)) continue;
+            s.wd.push_back(id(i, idx));
+            s.rd.push_back(id(i, idx));
+        }
+        break;
+    case 1: case 2: case 19: case 20: { // bc_convective
+        const bool column = (t == 1 || t == 2);
+        const int from = idx + ((t == 1 || t == 20) ? 1 : -1);
+        const int n = column ? Ny : Nx;
+        for (int e = 0; e < n; e++) {
+            const int xb = column ? idx : e, yb = column ? e : idx;
+            if (solid_h(L, xb, yb)) continue;
+            s.wd.push_back(id(xb, yb));
+            s.rd.push_back(column ? id(from, e) : id(e, from));
+        }
+        break;
+    }
+    case 5: case 6: {                   // bc_periodic
+        const bool along_x = (t == 5);
+        const int n = along_x ? Ny : Nx, len = along_x ? Nx : Ny;
+        for (int e = 0; e < n; e++)
+            for (int k = 1; k < 9; k++) {
+                const int c = along_x ? CX[k] : CY[k];
+                if (c == 0) continue;
+                const int dst = c > 0 ? 0 : len - 1, src = c > 0 ? len - c : -c - 1;
+                int i, j, is, js;
+                if (along_x) {
+                    i = dst; j = e; is = src; js = e - CY[k];
+                    if (js < 0 || js >= Ny) continue;
+                } else {
+                    i = e; j = dst; is = e - CX[k]; js = src;
+                    if (is < 0 || is >= Nx) continue;
+                }
+                if (solid_h(L, i, j) || solid_h(L, is, js)) continue;
+                s.wd.push_back(id(i, j));
+                s.rs.push_back(id(is, js));
+            }
+        break;
+    }
+    default: {                          // bc_regularized
+        const bool column = (t == 11 || t == 12 || t == 15 || t == 16);
+        const int step_in = (t == 12 || t == 14 || t == 16 || t == 18) ? 1 : -1;
+        const int n = column ? Ny : Nx;
+        for (int e = 0; e < n; e++) {
+            const int xb = column ? idx : e, yb = column ? e : idx;
+            if (solid_h(L, xb, yb)) continue;
+            const long long ni = column ? id(idx + step_in, e) : id(e, idx + step_in);
+            s.wd.push_back(id(xb, yb));
+            s.rd.push_back(ni);
+            s.rs.push_back(ni);
+        }
+        break;
+    }
+    }
+    for (auto *v : {&s.wd, &s.rd, &s.rs}) {
+        std::sort(v->begin(), v->end());
+        v->erase(std::unique(v->begin(), v->end()), v->end());
+    }
+    return s;
+}
+
+bool meet(const std::vector<long long> &a, const std::vector<long long> &b)
+{
+    size_t i = 0, j = 0;
+    while (i < a.size() && j < b.size()) {
+        if (a[i] == b[j]) return true;
+        if (a[i] < b[j]) i++;
+        else j++;
+    }
+    return false;
+}
+
+// The stale-edge job's nodes: fluid nodes of the non-periodic domain edges, whose
+// S slots and lagged moments edge_node rewrites.
+std::vector<long long> edge_nodes(const PlanGrid *L)
+{
+    const int Nx = L->Nx, Ny = L->Ny;
+    std::vector<long long> v;
+    auto add = [&](int x, int y) {
+        if (!solid_h(L, x, y)) v.push_back((long long)y * Nx + x);
+    };
+    if (!L->per_y)
+        for (int i = 0; i < Nx; i++) { add(i, 0); add(i, Ny - 1); }
+    if (!L->per_x)
+        for (int j = 0; j < Ny; j++) { add(0, j); add(Nx - 1, j); }
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    return v;
+}
+
+// Consecutive program entries share a launch while no entry writes a node that an
+// earlier one of the launch reads or writes, or reads a node that one writes; the
+// edge job (edges: the domain has a non-periodic axis) joins the last launch when
+// no entry there reads S on an edge node.
+std::vector<lbmx_lattice::TailLaunch> plan_tail(const PlanGrid *G, const int *type,
+                                                const int *index, int n, bool edges)
+{
+    std::vector<lbmx_lattice::TailLaunch> plan;
+    std::vector<NodeSets> cur;
+    lbmx_lattice::TailLaunch launch;
+    for (int b = 0; b < n; b++) {
+        NodeSets s = entry_nodes(G, type[b], index[b]);
+        bool clash = (int)launch.entry.size() >= kTailJobs - 1;
+        for (const NodeSets &o : cur) {
+            if (clash) break;
+            clash = meet(o.wd, s.wd) || meet(o.wd, s.rd) || meet(o.rd, s.wd);
+        }
+        if (clash) {
+            plan.push_back(launch);
+            launch = lbmx_lattice::TailLaunch{};
+            cur.clear();
+        }
+        launch.entry.push_back(b);
+        cur.push_back(std::move(s));
+    }
+    if (edges) {
+        const std::vector<long long> en = edge_nodes(G);
+        bool own_launch = launch.entry.empty();
+        for (const NodeSets &o : cur)
+            if (!own_launch) own_launch = meet(o.rs, en);
+        if (own_launch) {
+            if (!launch.entry.empty()) plan.push_back(launch);
+            launch = lbmx_lattice::TailLaunch{};
+        }
+        launch.edges = 1;
+    }
+    if (!launch.entry.empty() || launch.edges) plan.push_back(launch);
+    return plan;
+}
+}   // namespace
+
+// Leading periodic entries whose axis already wraps in streaming re-write exactly
+// what streaming wrote (periodic_x.c vs streaming.c:24-34): they are left out of
+// the program.  Returns the table indices of the program's entries.
+static std::vector<int> program_entries(const int *types, int nbc, int per_x, int per_y)
+{
+    std::vector<int> keep;
+    for (int b = 0; b < nbc; b++) {
+        const bool redundant = (types[b] == LBMX_BC_PERIODIC_X && per_x) ||
+                               (types[b] == LBMX_BC_PERIODIC_Y && per_y);
+        if (redundant && keep.empty()) continue;
+        keep.push_back(b);
+    }
+    return keep;
+}
+
+static void lbmx_plan_tail(lbmx_lattice *L)
+{
+    const PlanGrid G{L->Nx, L->Ny, L->cfg.per_x, L->cfg.per_y,
+                     L->hsolid.empty() ? nullptr : L->hsolid.data()};
+    L->tail = plan_tail(&G, L->prog_type.data(), L->prog_index.data(), L->nprog,
+                        L->edge_rows + L->edge_cols > 0);
+    L->tail_planned = 1;
+}
+
+extern "C" int lbmx_tail_plan(int Nx, int Ny, int per_x, int per_y, const int *mask,
+                              const int *types, const int *index, int nbc, int *launch_of,
+                              int *edge_launch)
+{
+    if (Nx < 3 || Ny < 3 || nbc < 0 || (nbc > 0 && (!types || !index || !launch_of)) ||
+        !edge_launch)
+        return lbmx_fail(LBMX_ERR_ARG, "invalid argument");
+    for (int b = 0; b < nbc; b++) {
+        int rc = lbmx_bc_check(types[b], index[b], Nx, Ny);
+        if (rc) return rc;
+    }
+    std::vector<uint8_t> solid;
+    if (mask) {
+        solid.resize((size_t)Nx * Ny);
+        for (size_t n = 0; n < solid.size(); n++) solid[n] = mask[n] ? 1 : 0;
+    }
+    const PlanGrid G{Nx, Ny, per_x ? 1 : 0, per_y ? 1 : 0, mask ? solid.data() : nullptr};
+    const std::vector<int> keep = program_entries(types, nbc, per_x, per_y);
+    std::vector<int> pt, pi;
+    for (int b : keep) {
+        pt.push_back(types[b]);
+        pi.push_back(index[b]);
+    }
+    const auto plan = plan_tail(&G, pt.data(), pi.data(), (int)keep.size(), !(per_x && per_y));
+    for (int b = 0; b < nbc; b++) launch_of[b] = -1;
+    *edge_launch = -1;
+    for (size_t l = 0; l < plan.size(); l++) {
+        for (int e : plan[l].entry) launch_of[keep[e]] = (int)l;
+        if (plan[l].edges) *edge_launch = (int)l;
+    }
+    return (int)plan.size();
+}
+
 template <typename T>
 static int launch_step(lbmx_lattice *L)
 {
@@ -712,63 +952,96 @@ static int launch_step(lbmx_lattice *L)
     int grid = L->bulk_grid;
     if ((long long)grid > ba.ntiles) grid = (int)((ba.ntiles + 7) / 8 * 8);
     if (grid < 8) grid = 8;
+
+    LinkArgs la;
+    la.g = g;
+    la.s = s;
+    la.F = F;
+    la.links = L->links;
+    la.nlinks = L->nlinks;
+    la.ibb = L->cfg.use_ibb;
+    la.write_bb = 0;
+    la.read_val = 0;
+    la.partials = L->partials;
+    // links in the bulk launch unless the bulk rewrites S (k_bulk_links)
+    const bool fused = L->nlinks > 0 && !s.exact;
+    const int link_grid = (L->npartials + L->nxcd - 1) / L->nxcd * L->nxcd;
+    const int gridA = grid + (fused ? link_grid : 0);
     switch (L->geom) {
-    case 0: hipLaunchKernelGGL((k_bulk<T, 0>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S, D, ba); break;
-    case 1: hipLaunchKernelGGL((k_bulk<T, 1>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S, D, ba); break;
-    default: hipLaunchKernelGGL((k_bulk<T, 2>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S, D, ba); break;
+    case 0:
+        hipLaunchKernelGGL((k_bulk<T, 0>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S, D, ba);
+        break;
+    case 1:
+        if (fused)
+            hipLaunchKernelGGL((k_bulk_links<T, 1>), dim3(gridA), dim3(kBulkThreads), 0,
+                               L->stream, S, D, ba, la, L->npartials, link_grid);
+        else
+            hipLaunchKernelGGL((k_bulk<T, 1>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S,
+                               D, ba);
+        break;
+    default:
+        if (fused)
+            hipLaunchKernelGGL((k_bulk_links<T, 2>), dim3(gridA), dim3(kBulkThreads), 0,
+                               L->stream, S, D, ba, la, L->npartials, link_grid);
+        else
+            hipLaunchKernelGGL((k_bulk<T, 2>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S,
+                               D, ba);
+        break;
     }
     HIPCHK(hipGetLastError());
-
-    if (L->nlinks) {
-        LinkArgs la;
-        la.g = g;
-        la.s = s;
-        la.F = F;
-        la.links = L->links;
-        la.nlinks = L->nlinks;
-        la.ibb = L->cfg.use_ibb;
-        la.write_bb = 0;
-        la.read_val = 0;
-        la.partials = L->partials;
+    if (L->nlinks && !fused) {
         hipLaunchKernelGGL(k_links<T>, dim3(L->npartials), dim3(kLinkThreads), 0, L->stream,
                            (const T *)S, D, la);
         HIPCHK(hipGetLastError());
     }
-    if (L->nprog || L->nlinks || L->frec) {
-        BcArgs bc;
-        bc.g = g;
-        bc.s = s;
-        bc.F = F;
-        bc.bcs = L->bc_dev;
-        bc.nbc = L->nprog;
-        bc.partials = L->partials;
-        bc.npartials = L->nlinks ? L->npartials : 0;
-        bc.force_out = L->force_dev;
-        bc.record = L->frec;
-        bc.row = L->frec_row++;
-        bc.capacity = L->frec_cap;
-        bc.dt = L->frec_dt;
-        bc.cf = L->frec_cf;
-        if (L->nprog == 0) {
-            hipLaunchKernelGGL(k_bc_one<T>, dim3(1), dim3(kBcOneThreads), 0, L->stream,
-                               (const T *)S, D, bc, 0, 1);
+
+    // boundary table, force outputs and stale edges: k_tail launches as planned
+    if (!L->tail_planned) lbmx_plan_tail(L);
+    TailArgs ta;
+    ta.bc.g = g;
+    ta.bc.s = s;
+    ta.bc.F = F;
+    ta.bc.bcs = L->bc_dev;
+    ta.bc.nbc = L->nprog;
+    ta.bc.partials = L->partials;
+    ta.bc.npartials = L->nlinks ? L->npartials : 0;
+    ta.bc.force_out = L->force_dev;
+    ta.bc.record = L->frec;
+    ta.bc.row = L->frec_row;
+    ta.bc.capacity = L->frec_cap;
+    ta.bc.dt = L->frec_dt;
+    ta.bc.cf = L->frec_cf;
+    ta.nrow = L->edge_rows;
+    ta.ncol = L->edge_cols;
+    bool force_due = L->nlinks > 0 || L->frec;
+    if (L->frec) L->frec_row++;
+    for (const auto &tl : L->tail) {
+        ta.njobs = 0;
+        ta.block0[0] = 0;
+        for (int b : tl.entry) {
+            ta.entry[ta.njobs] = b;
+            ta.block0[ta.njobs + 1] =
+                ta.block0[ta.njobs] + (L->prog_edge[b] + kTailThreads - 1) / kTailThreads;
+            ta.njobs++;
         }
-        for (int b = 0; b < L->nprog; b++) {
-            const int blocks = (L->prog_edge[b] + kBcOneThreads - 1) / kBcOneThreads;
-            hipLaunchKernelGGL(k_bc_one<T>, dim3(blocks), dim3(kBcOneThreads), 0, L->stream,
-                               (const T *)S, D, bc, b, b == 0 ? 1 : 0);
+        if (tl.edges && !s.exact) {
+            ta.entry[ta.njobs] = -1;
+            ta.block0[ta.njobs + 1] =
+                ta.block0[ta.njobs] + (ta.nrow + ta.ncol + kTailThreads - 1) / kTailThreads;
+            ta.njobs++;
         }
+        if (ta.njobs == 0) continue;
+        ta.with_force = force_due ? 1 : 0;
+        force_due = false;
+        hipLaunchKernelGGL(k_tail<T>, dim3(ta.block0[ta.njobs]), dim3(kTailThreads), 0,
+                           L->stream, S, D, ta);
         HIPCHK(hipGetLastError());
     }
-    if (!s.exact && (L->edge_rows + L->edge_cols) > 0) {
-        EdgeArgs ea;
-        ea.g = g;
-        ea.s = s;
-        ea.F = F;
-        ea.nrow = L->edge_rows;
-        ea.ncol = L->edge_cols;
-        const int ne = ea.nrow + ea.ncol;
-        hipLaunchKernelGGL(k_edges<T>, dim3((ne + 255) / 256), dim3(256), 0, L->stream, S, ea);
+    if (force_due) {   // links or a record but nothing else after streaming
+        ta.njobs = 0;
+        ta.block0[0] = 0;
+        ta.with_force = 1;
+        hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), 0, L->stream, S, D, ta);
         HIPCHK(hipGetLastError());
     }
     L->cur ^= 1;

@@ -61,6 +61,17 @@ struct lbmx_lattice {
     lbmx::BcDesc *bc_dev = nullptr;
     int nprog = 0;
     std::vector<int> prog_edge;   // edge length of each program entry
+    std::vector<int> prog_type, prog_index;
+    // host copy of the solid mask, Ny x Nx row-major (empty: no solid node)
+    std::vector<uint8_t> hsolid;
+    // the work after streaming as k_tail launches: program entries per launch and
+    // whether the launch also takes the stale-edge job (planned by lbmx_plan_tail)
+    struct TailLaunch {
+        std::vector<int> entry;
+        int edges = 0;
+    };
+    std::vector<TailLaunch> tail;
+    int tail_planned = 0;
 
     int has_geometry = 0, has_state = 0, lag_arrays = 0;
     double Fx_lag = 0.0, Fy_lag = 0.0;

@@ -10,10 +10,12 @@
 //               post-collision populations to the neighbours (streaming.c pull
 //               rules), half-way bounce-back into its own slots (bounce_back.c)
 //   k_links     solid links: interpolated bounce-back (interpolated_bounce_back.c)
-//               and momentum-exchange force partial sums (compute_force_mem.c)
-//   k_bc        the boundary table in order, one workgroup (main_lbm.c:49-58),
-//               then the fixed-order force total
-//   k_edges     stale-entry bookkeeping at non-periodic domain edges
+//               and momentum-exchange force partial sums (compute_force_mem.c);
+//               k_bulk_links runs both in one launch when S stays untouched
+//   k_tail      the boundary table in order (main_lbm.c:49-58), the fixed-order
+//               force total and the stale-entry bookkeeping at non-periodic
+//               domain edges, in as few launches as their dependences allow
+//               (k_bc and k_edges: the same work as stand-alone launches)
 // and k_moments produces compute_macroscopic_fields' output when it is asked for.
 //
 // Semantics reproduced exactly:
@@ -293,14 +295,13 @@ struct BulkArgs {
     int ibb;               // solid links are written by k_links (interpolated BB)
 };
 
+// Block b of a persistent bulk grid of G blocks.
 template <typename T, int GEOM>
-__global__ void __launch_bounds__(kBulkThreads) k_bulk(T *__restrict__ S, T *__restrict__ D,
-                                                      BulkArgs a)
+__device__ __forceinline__ void bulk_tiles(T *__restrict__ S, T *__restrict__ D,
+                                           const BulkArgs &a, int b, int G)
 {
     const Geo &g = a.g;
-    const int G = gridDim.x;
     const int per_xcd = G / a.nxcd;
-    const int b = blockIdx.x;
     const long long chunk = (long long)(b % a.nxcd) * per_xcd + b / a.nxcd;
     const long long t0 = a.ntiles * chunk / G, t1 = a.ntiles * (chunk + 1) / G;
     const size_t P = g.plane;
@@ -421,14 +422,11 @@ __global__ void __launch_bounds__(kBulkThreads) k_bulk(T *__restrict__ S, T *__r
     }
 }
 
-
-// Select element k (runtime) of a small register array without scratch.
-__device__ __forceinline__ double sel9(const double a[Q], int k)
+template <typename T, int GEOM>
+__global__ void __launch_bounds__(kBulkThreads) k_bulk(T *__restrict__ S, T *__restrict__ D,
+                                                      BulkArgs a)
 {
-    double r = a[0];
-#pragma unroll
-    for (int i = 1; i < Q; i++) r = (k == i) ? a[i] : r;
-    return r;
+    bulk_tiles<T, GEOM>(S, D, a, blockIdx.x, gridDim.x);
 }
 
 // ---- k_links -----------------------------------------------------------------
@@ -481,31 +479,53 @@ __device__ __forceinline__ void block_sum2(double &fx, double &fy, double *lds)
     }
 }
 
+// f*(x, K) and the value (interpolated) bounce-back gives f_new(x, opp K), for a
+// link in direction K; K is a compile-time constant so both post-collision node
+// arrays stay in registers.
+template <int K, typename T>
+__device__ __forceinline__ void link_value(const T *__restrict__ S, const LinkArgs &a,
+                                           const Link &L, double &fk, double &val)
+{
+    constexpr int KO = D2Q9::opp[K];
+    double st[Q];
+    star_node(S, a.g, a.F, a.s, (size_t)L.n, st);
+    fk = st[K];
+    val = fk;
+    if (a.ibb && !a.read_val && L.nb >= 0) {
+        const double q = L.q;
+        double sb[Q];
+        star_node(S, a.g, a.F, a.s, (size_t)L.nb, sb);
+        if (q <= 0.5)
+            val = (1.0 - 2.0 * q) * fk + 2.0 * q * sb[K];
+        else if (!L.bsolid)
+            val = (1.0 / (2.0 * q)) * fk + (2.0 * q - 1.0) / (2.0 * q) * sb[KO];
+    }
+}
+
+// Links [blk * kLinkThreads, +kLinkThreads) and their force partial sum, which goes
+// to partials[blk]; the partials are summed in index order by bc_force_total.
 template <typename T>
-__global__ void __launch_bounds__(kLinkThreads) k_links(const T *__restrict__ S,
-                                                        T *__restrict__ D, LinkArgs a)
+__device__ __forceinline__ void links_block(const T *__restrict__ S, T *__restrict__ D,
+                                            const LinkArgs &a, int blk)
 {
     __shared__ double lds[2 * kLinkThreads / 64];
-    const int l = blockIdx.x * kLinkThreads + threadIdx.x;
+    const int l = blk * kLinkThreads + threadIdx.x;
     double fx = 0.0, fy = 0.0;
     if (l < a.nlinks) {
         const Link L = a.links[l];
         const int k = L.k, ko = c_opp[k];
-        double st[Q];
-        star_node(S, a.g, a.F, a.s, (size_t)L.n, st);
-        const double fk = sel9(st, k);
-        double val = fk;
-        if (a.read_val) {
-            val = pop_get(D, (size_t)ko * a.g.plane + (size_t)L.n, ko, a.g.rho0);
-        } else if (a.ibb && L.nb >= 0) {
-            const double q = L.q;
-            double sb[Q];
-            star_node(S, a.g, a.F, a.s, (size_t)L.nb, sb);
-            if (q <= 0.5)
-                val = (1.0 - 2.0 * q) * fk + 2.0 * q * sel9(sb, k);
-            else if (!L.bsolid)
-                val = (1.0 / (2.0 * q)) * fk + (2.0 * q - 1.0) / (2.0 * q) * sel9(sb, ko);
+        double fk, val;
+        switch (k) {
+        case 1: link_value<1>(S, a, L, fk, val); break;
+        case 2: link_value<2>(S, a, L, fk, val); break;
+        case 3: link_value<3>(S, a, L, fk, val); break;
+        case 4: link_value<4>(S, a, L, fk, val); break;
+        case 5: link_value<5>(S, a, L, fk, val); break;
+        case 6: link_value<6>(S, a, L, fk, val); break;
+        case 7: link_value<7>(S, a, L, fk, val); break;
+        default: link_value<8>(S, a, L, fk, val); break;
         }
+        if (a.read_val) val = pop_get(D, (size_t)ko * a.g.plane + (size_t)L.n, ko, a.g.rho0);
         if ((a.ibb || a.write_bb) && !a.read_val)
             pop_put(D, (size_t)ko * a.g.plane + (size_t)L.n, ko, val, a.g.rho0);
         // compute_force_mem.c:55-57: f*(x,k) + f_new(x,opp k) after (I)BB
@@ -515,9 +535,39 @@ __global__ void __launch_bounds__(kLinkThreads) k_links(const T *__restrict__ S,
     }
     block_sum2(fx, fy, lds);
     if (threadIdx.x == 0) {
-        a.partials[2 * blockIdx.x] = fx;
-        a.partials[2 * blockIdx.x + 1] = fy;
+        a.partials[2 * blk] = fx;
+        a.partials[2 * blk + 1] = fy;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kLinkThreads) k_links(const T *__restrict__ S,
+                                                        T *__restrict__ D, LinkArgs a)
+{
+    links_block(S, D, a, blockIdx.x);
+}
+
+// k_bulk and k_links in one launch.  The link blocks come first, so their
+// latency-bound work starts at once and overlaps the bulk: blocks [0, nlink) run
+// the links, [nlink, link_grid) are padding that keeps the bulk's blocks on the
+// XCDs its tile mapping assumes (link_grid is a multiple of the XCD count), and
+// the blocks after link_grid run the bulk tiles.  Valid when S is not written during the step (not
+// LBMX_EXACT_STATE): the links then read only S, and a link slot D(opp k, n) is
+// written either by the bulk (half-way bounce-back, no interpolation) or by the
+// link blocks (interpolated bounce-back), never by both -- streaming skips its
+// solid source n + c_k -- so the two sets of blocks are independent.
+template <typename T, int GEOM>
+__global__ void __launch_bounds__(kBulkThreads) k_bulk_links(T *__restrict__ S,
+                                                            T *__restrict__ D, BulkArgs ab,
+                                                            LinkArgs al, int nlink,
+                                                            int link_grid)
+{
+    static_assert(kBulkThreads == kLinkThreads, "one block size for both parts");
+    const int b = blockIdx.x;
+    if (b < nlink)
+        links_block(S, D, al, b);
+    else if (b >= link_grid)
+        bulk_tiles<T, GEOM>(S, D, ab, b - link_grid, (int)gridDim.x - link_grid);
 }
 
 // ---- k_bc: the boundary table, in order, in one workgroup ----------------------
@@ -543,13 +593,15 @@ struct BcArgs {
     double dt, cf;       // t = row * dt, L and D scaled by cf
 };
 
-// Edge nodes of one boundary are independent of each other: the threads of every
-// workgroup of the launch share them (one workgroup in k_bc, many in k_bc_one).
-__device__ __forceinline__ int bc_first() { return blockIdx.x * blockDim.x + threadIdx.x; }
-__device__ __forceinline__ int bc_stride() { return gridDim.x * blockDim.x; }
+// Edge nodes of one boundary are independent of each other: a boundary is swept by
+// the threads given in a Span (one workgroup in k_bc, several in k_tail), thread
+// `first` taking nodes first, first + stride, ...
+struct Span {
+    int first, stride;
+};
 
 template <typename T>
-__device__ void bc_zou_he_column(const T *S, T *D, const BcArgs &a, const BcDesc &d)
+__device__ void bc_zou_he_column(const T *S, T *D, const BcArgs &a, const BcDesc &d, Span sp)
 {
     // velocity_inlet.c:28-39, velocity_outlet.c:28-39, pressure_inlet.c:25-39,
     // pressure_outlet.c:25-39
@@ -558,7 +610,7 @@ __device__ void bc_zou_he_column(const T *S, T *D, const BcArgs &a, const BcDesc
     const int i = d.index;
     const bool inlet = (d.type == 7 || d.type == 3);
     const bool pressure = (d.type == 3 || d.type == 4);
-    for (int j = bc_first(); j < g.Ny; j += bc_stride()) {
+    for (int j = sp.first; j < g.Ny; j += sp.stride) {
         const size_t n = (size_t)j * g.pitch + i;
         if (solid_at(g, n)) continue;
         double f[Q];
@@ -647,7 +699,7 @@ __device__ __forceinline__ void regularized_node(const double fi[Q], double ri, 
 }
 
 template <typename T>
-__device__ void bc_zou_he_row(T *D, const BcArgs &a, const BcDesc &d)
+__device__ void bc_zou_he_row(T *D, const BcArgs &a, const BcDesc &d, Span sp)
 {
     // velocity_top.c:28-39 and velocity_bottom.c:28-39 (the latter stores its
     // third closure into slot 7, leaving slot 6 as streamed)
@@ -655,7 +707,7 @@ __device__ void bc_zou_he_row(T *D, const BcArgs &a, const BcDesc &d)
     const size_t P = g.plane;
     const int j = d.index;
     const bool top = (d.type == 9);
-    for (int i = bc_first(); i < g.Nx; i += bc_stride()) {
+    for (int i = sp.first; i < g.Nx; i += sp.stride) {
         const size_t n = (size_t)j * g.pitch + i;
         if (solid_at(g, n)) continue;
         const double U = d.v1[i], V = d.v2[i];
@@ -679,7 +731,7 @@ __device__ void bc_zou_he_row(T *D, const BcArgs &a, const BcDesc &d)
 }
 
 template <typename T>
-__device__ void bc_regularized(const T *S, T *D, const BcArgs &a, const BcDesc &d)
+__device__ void bc_regularized(const T *S, T *D, const BcArgs &a, const BcDesc &d, Span sp)
 {
     // src/bc/*_regularized.c:18-71: feq(rho_b, u_b) + 1.5 w Q:Pi with Pi from
     // f_new - feq(lagged) at the neighbouring interior node
@@ -690,7 +742,7 @@ __device__ void bc_regularized(const T *S, T *D, const BcArgs &a, const BcDesc &
     const bool pressure = (t >= 15 && t <= 18);
     const int step_in = (t == 12 || t == 14 || t == 16 || t == 18) ? 1 : -1;
     const int nedge = column ? g.Ny : g.Nx;
-    for (int e = bc_first(); e < nedge; e += bc_stride()) {
+    for (int e = sp.first; e < nedge; e += sp.stride) {
         const int ib = column ? d.index : e, jb = column ? e : d.index;
         const int ii = column ? d.index + step_in : e, ji = column ? e : d.index + step_in;
         const size_t nb = (size_t)jb * g.pitch + ib, ni = (size_t)ji * g.pitch + ii;
@@ -717,7 +769,7 @@ __device__ void bc_regularized(const T *S, T *D, const BcArgs &a, const BcDesc &
 }
 
 template <typename T>
-__device__ void bc_convective(T *D, const BcArgs &a, const BcDesc &d)
+__device__ void bc_convective(T *D, const BcArgs &a, const BcDesc &d, Span sp)
 {
     // convective_{inlet,outlet,top,bottom}.c:18-28: copy all nine f_new
     const Geo &g = a.g;
@@ -726,7 +778,7 @@ __device__ void bc_convective(T *D, const BcArgs &a, const BcDesc &d)
     const bool column = (t == 1 || t == 2);
     const int from = d.index + ((t == 1 || t == 20) ? 1 : -1);
     const int nedge = column ? g.Ny : g.Nx;
-    for (int e = bc_first(); e < nedge; e += bc_stride()) {
+    for (int e = sp.first; e < nedge; e += sp.stride) {
         const size_t nb = column ? (size_t)e * g.pitch + d.index : (size_t)d.index * g.pitch + e;
         const size_t ns = column ? (size_t)e * g.pitch + from : (size_t)from * g.pitch + e;
         if (solid_at(g, nb)) continue;
@@ -736,7 +788,7 @@ __device__ void bc_convective(T *D, const BcArgs &a, const BcDesc &d)
 }
 
 template <typename T>
-__device__ void bc_periodic(const T *S, T *D, const BcArgs &a, bool along_x)
+__device__ void bc_periodic(const T *S, T *D, const BcArgs &a, bool along_x, Span sp)
 {
     // periodic_x.c:19-50 / periodic_y.c:19-50: re-pull wrapped populations of the
     // first/last column (row) from post-collision f; the other axis is not wrapped
@@ -744,7 +796,7 @@ __device__ void bc_periodic(const T *S, T *D, const BcArgs &a, bool along_x)
     const size_t P = g.plane;
     const int nedge = along_x ? g.Ny : g.Nx;
     const int L = along_x ? g.Nx : g.Ny;
-    for (int e = bc_first(); e < nedge; e += bc_stride()) {
+    for (int e = sp.first; e < nedge; e += sp.stride) {
 #pragma unroll
         for (int k = 1; k < Q; k++) {
             const int c = along_x ? D2Q9::cx[k] : D2Q9::cy[k];
@@ -769,15 +821,16 @@ __device__ void bc_periodic(const T *S, T *D, const BcArgs &a, bool along_x)
 }
 
 template <typename T>
-__device__ __forceinline__ void bc_apply(const T *S, T *D, const BcArgs &a, const BcDesc &d)
+__device__ __forceinline__ void bc_apply(const T *S, T *D, const BcArgs &a, const BcDesc &d,
+                                         Span sp)
 {
     switch (d.type) {
-    case 3: case 4: case 7: case 8: bc_zou_he_column(S, D, a, d); break;
-    case 9: case 10: bc_zou_he_row(D, a, d); break;
-    case 1: case 2: case 19: case 20: bc_convective(D, a, d); break;
-    case 5: bc_periodic(S, D, a, true); break;
-    case 6: bc_periodic(S, D, a, false); break;
-    default: bc_regularized(S, D, a, d); break;
+    case 3: case 4: case 7: case 8: bc_zou_he_column(S, D, a, d, sp); break;
+    case 9: case 10: bc_zou_he_row(D, a, d, sp); break;
+    case 1: case 2: case 19: case 20: bc_convective(D, a, d, sp); break;
+    case 5: bc_periodic(S, D, a, true, sp); break;
+    case 6: bc_periodic(S, D, a, false, sp); break;
+    default: bc_regularized(S, D, a, d, sp); break;
     }
 }
 
@@ -808,25 +861,10 @@ __global__ void __launch_bounds__(kBcThreads) k_bc(const T *__restrict__ S, T *_
                                                    BcArgs a)
 {
     for (int b = 0; b < a.nbc; b++) {
-        bc_apply(S, D, a, a.bcs[b]);
+        bc_apply(S, D, a, a.bcs[b], Span{(int)threadIdx.x, (int)blockDim.x});
         __syncthreads();
     }
     if (threadIdx.x == 0) bc_force_total(a);
-}
-
-// Table entry b over as many workgroups as its edge needs; the entries run as
-// successive launches on one stream, so each sees its predecessors' writes
-// (corners) exactly as in main_lbm.c:49-58.  The first launch also forms the
-// force outputs (the link partials are complete when it starts).
-constexpr int kBcOneThreads = 128;
-
-template <typename T>
-__global__ void __launch_bounds__(kBcOneThreads) k_bc_one(const T *__restrict__ S,
-                                                          T *__restrict__ D, BcArgs a, int b,
-                                                          int with_force)
-{
-    if (b < a.nbc) bc_apply(S, D, a, a.bcs[b]);
-    if (with_force && blockIdx.x == 0 && threadIdx.x == 0) bc_force_total(a);
 }
 
 // ---- k_edges: stale slots of non-periodic domain edges ----------------------------
@@ -842,10 +880,9 @@ struct EdgeArgs {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_edges(T *__restrict__ S, EdgeArgs a)
+__device__ __forceinline__ void edge_node(T *__restrict__ S, const EdgeArgs &a, int e)
 {
     const Geo &g = a.g;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.nrow + a.ncol) return;
     int x, y;
     if (e < a.nrow) {   // rows 0 and Ny-1 when y is not periodic
@@ -876,6 +913,53 @@ __global__ void __launch_bounds__(256) k_edges(T *__restrict__ S, EdgeArgs a)
     a.F.R[n] = m.rho;
     a.F.U[n] = m.ux;
     a.F.V[n] = m.uy;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_edges(T *__restrict__ S, EdgeArgs a)
+{
+    edge_node(S, a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---- k_tail: the work after streaming, in as few launches as its order allows ------
+// A launch holds jobs -- boundary table entries, or the stale-edge bookkeeping --
+// that the host has found independent of each other: no job writes a node another
+// job of the launch reads or writes (corners of two boundaries), and the edge job,
+// which rewrites S, is only given a launch whose boundaries read S elsewhere.
+// Launches run in stream order, so an entry still sees every write of the entries
+// before it (main_lbm.c:49-58).  Blocks [block0[j], block0[j+1]) sweep job j; block 0
+// of the first launch also forms the force outputs (the link partials are complete
+// when it starts).
+constexpr int kTailThreads = 128;
+constexpr int kTailJobs = 8;
+
+struct TailArgs {
+    BcArgs bc;
+    int nrow, ncol;                 // the edge job's node counts (EdgeArgs)
+    int njobs;
+    int with_force;
+    int entry[kTailJobs];           // boundary table entry, or -1 for the edge job
+    int block0[kTailJobs + 1];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kTailThreads) k_tail(T *__restrict__ S, T *__restrict__ D,
+                                                      TailArgs a)
+{
+    const int blk = blockIdx.x;
+    if (a.njobs > 0) {
+        int j = 0;
+        while (j + 1 < a.njobs && blk >= a.block0[j + 1]) j++;
+        const Span sp{(blk - a.block0[j]) * kTailThreads + (int)threadIdx.x,
+                      (a.block0[j + 1] - a.block0[j]) * kTailThreads};
+        if (a.entry[j] >= 0) {
+            bc_apply((const T *)S, D, a.bc, a.bc.bcs[a.entry[j]], sp);
+        } else {
+            const EdgeArgs ea{a.bc.g, a.bc.s, a.bc.F, a.nrow, a.ncol};
+            for (int e = sp.first; e < a.nrow + a.ncol; e += sp.stride) edge_node(S, ea, e);
+        }
+    }
+    if (a.with_force && blk == 0 && threadIdx.x == 0) bc_force_total(a.bc);
 }
 
 // ---- k_materialize: the reference's post-collision buffer, on demand ----------------

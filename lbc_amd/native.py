@@ -101,6 +101,7 @@ def _bind():
     L.lbmx_steps_done.restype = C.c_int64
     L.lbmx_device_bytes.argtypes = [vp]
     L.lbmx_device_bytes.restype = C.c_int64
+    L.lbmx_tail_plan.argtypes = [i, i, i, i, _ip, _ip, _ip, i, _ip, _ip]
     _bound = True
     return L
 
@@ -132,6 +133,24 @@ def mei_initialization(f, f_new, rho_in, U_in, V_in, rho, u, v, omega, per_x, pe
     if n < 0:
         check(n)
     return n
+
+
+def tail_plan(Nx, Ny, per_x, per_y, mask, bcs):
+    """How lbmx_step groups the boundary table and the stale-edge bookkeeping into
+    launches (lbmx_tail_plan; host only).  bcs: [(name, index), ...].  Returns
+    (nlaunch, launch_of, edge_launch): launch_of[b] is the launch of entry b (-1:
+    left out), edge_launch that of the edge job (-1: none)."""
+    L = _bind()
+    t = np.array([BC_CODES[b[0]] for b in bcs] or [0], dtype=np.int32)
+    ix = np.array([int(b[1]) for b in bcs] or [0], dtype=np.int32)
+    out = np.zeros(max(1, len(bcs)), dtype=np.int32)
+    edge = np.zeros(1, dtype=np.int32)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
+    n = L.lbmx_tail_plan(int(Nx), int(Ny), int(per_x), int(per_y), _i(m), _i(t), _i(ix),
+                         len(bcs), _i(out), _i(edge))
+    if n < 0:
+        check(n)
+    return n, out[:len(bcs)].tolist(), int(edge[0])
 
 
 class Lattice:
