@@ -16,6 +16,9 @@ Also reported in the same JSON line:
                 sources with its flags) timed on this host through main_lbm on a
                 bounded sample of the same workload
   secondary     FP32-storage MLUPS at 16384^2 (FP32 collision and FP64 collision),
+                a plain device copy's GB/s (the two-buffer ceiling), the roofline
+                sweep 1024^2 .. 16384^2 in FP64 and FP32, the 256^2 Taylor-Green
+                case against the analytic decay,
                 the FP64 cylinder channel 2048 x 512 (interpolated bounce-back,
                 regularized inlet/outlet), the batched workload (1,024 lattices x
                 5,000 steps) and the serving sweep (Poisson arrivals of liner-duct
@@ -62,6 +65,68 @@ def time_lattice(lat, steps, warmup):
     ms = lat.time_steps(steps)
     lat.synchronize()
     return time.perf_counter() - t0, ms
+
+
+INFINITY_CACHE_BYTES = 256 << 20
+
+
+def roofline_sweep(sizes=(1024, 2048, 4096, 8192, 16384), seconds=2.0):
+    """Periodic Taylor-Green step at each N in FP64 and FP32 storage (FP32
+    collision), each point timed for at least `seconds`; moments are not read
+    inside the timed region.  Points whose two population buffers fit the 256 MB
+    Infinity Cache are labelled: their rate is not an HBM rate."""
+    from lbc_amd import native
+    rows = []
+    for N in sizes:
+        for name, prec, bpc in (("fp64", native.F64, 144.0), ("fp32", native.F32_SHIFTED, 72.0)):
+            lat = tgv_lattice(N, prec)
+            _, probe_ms = time_lattice(lat, 20, 10)
+            steps = max(50, int(math.ceil(seconds * 1e3 / max(probe_ms / 20, 1e-6))))
+            wall, ms = time_lattice(lat, steps, 10)
+            lat.close()
+            cells = N * N
+            gbs = bpc * cells / (ms / steps * 1e-3) / 1e9
+            rows.append({"N": N, "precision": name, "steps": steps,
+                         "MLUPS": round(cells * steps / wall / 1e6, 1),
+                         "GB_per_s": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                         # two buffers of bpc / 2 bytes per cell
+                         "cache_resident": bool(bpc * cells <= INFINITY_CACHE_BYTES)})
+            log(f"sweep {name} {N}^2: {rows[-1]['MLUPS']:.0f} MLUPS, {gbs:.0f} GB/s "
+                f"({steps} steps{', cache resident' if rows[-1]['cache_resident'] else ''})")
+    return {"workload": "taylor_green_periodic_NxN_roofline_sweep", "points": rows}
+
+
+def copy_ceiling(gib=8.0):
+    """GB/s (bytes read + written) of a plain device copy of `gib` GiB with 16-byte
+    loads and non-temporal stores (lbmx_copy_rate): the practical ceiling a
+    two-buffer stencil step is held against."""
+    from lbc_amd import native
+    return native.copy_rate(gib)
+
+
+def tgv256_bench(steps=10000):
+    """The reference's own Taylor-Green case, 256 x 256, FP64, tau = 0.8, u0 = 0.01,
+    `steps` steps: rate and the relative L2 deviation of (u, v) from the decaying
+    analytic vortex u0 exp(-2 nu k^2 t)."""
+    from lbc_amd import native
+    N, u0, tau = 256, 0.01, 0.8
+    k, nu = 2.0 * math.pi / N, (tau - 0.5) / 3.0
+    lat = tgv_lattice(N, native.F64)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    lat.step(steps)
+    lat.synchronize()
+    wall = time.perf_counter() - t0
+    _, u, v = lat.moments()
+    lat.close()
+    j, i = np.mgrid[0:N, 0:N].astype(np.float64)
+    damp = math.exp(-2.0 * nu * k * k * steps)
+    ua = u0 * np.sin(k * i) * np.cos(k * j) * damp
+    va = -u0 * np.cos(k * i) * np.sin(k * j) * damp
+    err = math.sqrt((((u - ua) ** 2 + (v - va) ** 2).sum()) / ((ua ** 2 + va ** 2).sum()))
+    return {"workload": f"taylor_green_periodic_256x256_fp64_{steps}_steps",
+            "MLUPS": round(N * N * steps / wall / 1e6, 1), "seconds": round(wall, 3),
+            "l2_velocity_error_vs_analytic": float(f"{err:.3e}")}
 
 
 def cylinder_lattice(precision):
@@ -413,6 +478,8 @@ def main():
     ap.add_argument("--serving", type=int, default=32, metavar="N",
                     help="requests per arrival rate of the serving sweep (0: skip)")
     ap.add_argument("--cylinder-steps", type=int, default=50000)
+    ap.add_argument("--no-sweep", dest="sweep", action="store_false",
+                    help="skip the Taylor-Green roofline sweep 1024^2 .. 16384^2")
     ap.add_argument("--liner", type=int, default=20000, metavar="STEPS",
                     help="steps of the 8192 x 256 liner duct in FP64 and FP32 (0: skip)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
@@ -471,6 +538,14 @@ def main():
             "GB_per_s": round(72.0 * cells / (dw / args.steps * 1e-3) / 1e9, 1),
             "vs_fp64": round(mw / mlups, 3)}
         log(f"FP32 storage, FP64 collision {N}^2: {mw:.0f} MLUPS ({mw / mlups:.2f}x FP64)")
+        copy = copy_ceiling()
+        secondary["copy_ceiling"] = {"GB_per_s": round(copy, 1),
+                                     "fp64_step_frac_of_copy": round(achieved / copy, 4)}
+        log(f"copy ceiling {copy:.0f} GB/s; the FP64 step reaches {achieved / copy:.1%} of it")
+        if args.sweep:
+            secondary["roofline_sweep"] = roofline_sweep()
+        secondary["tgv256"] = tgv256_bench()
+        log(f"TGV 256^2: {secondary['tgv256']}")
         secondary["cylinder"] = cylinder_bench(args.cylinder_steps)
     if args.batched > 0:
         secondary["batched"] = batched_bench(args.batched, args.batched_steps,

@@ -214,6 +214,12 @@ LBMX_API int lbmx_tail_plan(int Nx, int Ny, int per_x, int per_y, const int *mas
                             const int *types, const int *index, int nbc, int *launch_of,
                             int *edge_launch);
 
+/* Measured ceiling of a two-buffer step (no counterpart in the reference): GB/s
+ * (bytes read + written) of a plain device copy of gib GiB, 16-byte loads and
+ * non-temporal stores, 5 timed launches after 2 untimed ones.  Allocates and frees
+ * two buffers of gib GiB. */
+LBMX_API int lbmx_copy_rate(double gib, double *gbs);
+
 /* Mei consistent initialization on the device (mei_initialization.c:7-142):
  * same arguments and in/out contract as the reference function with the
  * lattice tables implied.  f/f_new are read as the starting buffers and

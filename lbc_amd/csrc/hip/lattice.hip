@@ -50,6 +50,53 @@ extern "C" const char *lbmx_device_name(void)
     return name.c_str();
 }
 
+// ---------------------------------------------------------------------------
+// copy ceiling: one 16-byte load and one non-temporal 16-byte store per thread
+// ---------------------------------------------------------------------------
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+__global__ void __launch_bounds__(256) k_copy16(const dbl2 *__restrict__ a,
+                                                dbl2 *__restrict__ b, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(a[i], &b[i]);
+}
+
+extern "C" int lbmx_copy_rate(double gib, double *gbs)
+{
+    if (!gbs || !(gib > 0.0) || gib > 64.0) return lbmx_fail(LBMX_ERR_ARG, "gib must be in (0, 64]");
+    const long long n = (long long)(gib * (double)(1ll << 30)) / 16 / 256 * 256;
+    if (n <= 0) return lbmx_fail(LBMX_ERR_ARG, "gib too small");
+    dbl2 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = LBMX_OK;
+    auto run = [&]() -> int {
+        HIPCHK(hipMalloc((void **)&a, n * sizeof(dbl2)));
+        HIPCHK(hipMalloc((void **)&b, n * sizeof(dbl2)));
+        HIPCHK(hipMemset(a, 0, n * sizeof(dbl2)));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const unsigned grid = (unsigned)(n / 256);
+        const int reps = 5;
+        for (int r = 0; r < 2; r++) hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, 0, a, b, n);
+        HIPCHK(hipEventRecord(e0, 0));
+        for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, 0, a, b, n);
+        HIPCHK(hipEventRecord(e1, 0));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipGetLastError());
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        *gbs = 2.0 * 16.0 * (double)n * reps / ((double)ms * 1e-3) / 1e9;
+        return LBMX_OK;
+    };
+    rc = run();
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return rc;
+}
+
 int lbmx_device_info(int *ncu)
 {
     static int cached = 0;

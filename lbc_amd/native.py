@@ -102,6 +102,7 @@ def _bind():
     L.lbmx_device_bytes.argtypes = [vp]
     L.lbmx_device_bytes.restype = C.c_int64
     L.lbmx_tail_plan.argtypes = [i, i, i, i, _ip, _ip, _ip, i, _ip, _ip]
+    L.lbmx_copy_rate.argtypes = [d, _dp]
     _bound = True
     return L
 
@@ -133,6 +134,13 @@ def mei_initialization(f, f_new, rho_in, U_in, V_in, rho, u, v, omega, per_x, pe
     if n < 0:
         check(n)
     return n
+
+
+def copy_rate(gib=8.0):
+    """GB/s (read + written) of a plain device copy of `gib` GiB (lbmx_copy_rate)."""
+    out = C.c_double()
+    check(_bind().lbmx_copy_rate(float(gib), C.byref(out)))
+    return out.value
 
 
 def tail_plan(Nx, Ny, per_x, per_y, mask, bcs):

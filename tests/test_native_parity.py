@@ -188,3 +188,86 @@ def test_force_record_matches_oracle(sc):
     assert np.array_equal(rec[:, 0], np.arange(NSTEPS) * dt)
     np.testing.assert_allclose(rec[:, 1], ref_forces[:, 0] * cf, rtol=1e-11, atol=1e-12)
     np.testing.assert_allclose(rec[:, 2], ref_forces[:, 1] * cf, rtol=1e-11, atol=1e-12)
+
+
+PUSH_LONG = [s for s in all_scenarios(2) if s.name in (
+    "cylinder_ibb", "channel_regularized_ibb", "testbc_regularized", "inner_index_bcs",
+    "zouhe_box", "ldc_walls")]
+
+
+@pytest.mark.parametrize("sc", PUSH_LONG, ids=[s.name for s in PUSH_LONG])
+def test_push_multistep_bit_identical(sc):
+    """200 steps in one lbmx_step call on the push path (bulk and links in one
+    launch, the boundary table and stale edges in planned launches), fixed
+    boundary values: bit-identical to the oracle."""
+    from lbc_amd import native
+    nsteps = 200
+    state = sc.initial_state()
+    vals = sc.vals(0)
+    lat0 = oracle_lattice(sc, state)
+    O.oracle_run(lat0, nsteps, vals, sc.per_x, sc.per_y, sc.Fx, sc.Fy, sc.use_ibb)
+    lat = native.Lattice(sc.Nx, sc.Ny, precision=0, per_x=sc.per_x, per_y=sc.per_y,
+                         use_ibb=sc.use_ibb, Fx=sc.Fx, Fy=sc.Fy)
+    lat.set_geometry(sc.mask, sc.omega, sc.phi)
+    lat.set_boundaries(sc.bc_names)
+    for b, (_, _, v1, v2) in enumerate(vals):
+        lat.set_bc_values(b, v1, v2)
+    lat.upload(*state)
+    lat.step(nsteps)
+    rho, u, v = lat.moments()
+    f = lat.download()
+    lat.close()
+    assert same_bits(rho, lat0.rho) and same_bits(u, lat0.u) and same_bits(v, lat0.v)
+    assert same_bits(f, lat0.f)
+
+
+def tgv256_state(N=256, u0=0.01):
+    """Taylor-Green vortex, k = 2 pi / N, x = i, y = j; rho' = p'/cs^2."""
+    from cases import feq
+    k = 2 * np.pi / N
+    j, i = np.mgrid[0:N, 0:N].astype(np.float64)
+    u = u0 * np.sin(k * i) * np.cos(k * j)
+    v = -u0 * np.cos(k * i) * np.sin(k * j)
+    rho = 1 - 3 * u0 ** 2 / 4 * (np.cos(2 * k * i) + np.cos(2 * k * j))
+    f = feq(rho, u, v)
+    return f, f.copy(), rho, u, v
+
+
+def tgv256_gpu(nsteps):
+    from lbc_amd import native
+    N = 256
+    lat = native.Lattice(N, N, precision=0, per_x=1, per_y=1)
+    lat.set_uniform(1.0 / 0.8)
+    lat.set_boundaries([("periodic_x", 0), ("periodic_y", 0)])
+    lat.upload(*tgv256_state())
+    lat.step(nsteps)
+    out = lat.moments()
+    lat.close()
+    return out
+
+
+def test_tgv256_bit_identical():
+    """The reference's own Taylor-Green case (256 x 256, tau = 0.8, u0 = 0.01) for
+    500 steps in one call (gather form): equal to the oracle bit for bit."""
+    N = 256
+    lat0 = O.Lattice(N, N)
+    lat0.f, lat0.f_new, lat0.rho, lat0.u, lat0.v = tgv256_state()
+    lat0.omega[:] = 1.0 / 0.8
+    O.oracle_run(lat0, 500, [("periodic_x", 0, None, None), ("periodic_y", 0, None, None)], 1, 1)
+    rho, u, v = tgv256_gpu(500)
+    assert same_bits(rho, lat0.rho) and same_bits(u, lat0.u) and same_bits(v, lat0.v)
+
+
+def test_tgv256_analytic_decay():
+    """10,000 steps of the same case against u0 exp(-2 nu k^2 t): relative L2 velocity
+    deviation below 3e-3 (the oracle's own deviation is 2.23e-3: O(k^2) lattice
+    error plus the start from equilibrium populations)."""
+    N, u0, nu, T = 256, 0.01, 0.1, 10000
+    k = 2 * np.pi / N
+    _, u, v = tgv256_gpu(T)
+    j, i = np.mgrid[0:N, 0:N].astype(np.float64)
+    d = np.exp(-2 * nu * k * k * T)
+    ua, va = u0 * np.sin(k * i) * np.cos(k * j) * d, -u0 * np.cos(k * i) * np.sin(k * j) * d
+    err = np.sqrt((((u - ua) ** 2 + (v - va) ** 2).sum()) / ((ua ** 2 + va ** 2).sum()))
+    print(f"TGV 256^2, {T} steps: L2 deviation from the analytic vortex {err:.3e}")
+    assert err < 3e-3
