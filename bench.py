@@ -319,6 +319,29 @@ def serving_bench(rates=(10, 30, 100, 300, 1000), n=32, seq_sample=6, seed=2026,
     }
 
 
+def vtk_output_times(lat, reps=3):
+    """One VTK output step of the lattice: the device-packed writer
+    (lbmx_write_vtk) against downloading rho/u/v and the host writer; seconds
+    (best of reps) and whether the two files are identical."""
+    import tempfile
+    from lbc_amd import native
+    with tempfile.TemporaryDirectory() as d:
+        a, b = os.path.join(d, "dev.vtk"), os.path.join(d, "host.vtk")
+        td, th = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            lat.write_vtk(a, 1.0, 1.0, 1.0)
+            td.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            rho, u, v = lat.moments()
+            native.write_vtk_host(b, 1.0, u, v, rho)
+            th.append(time.perf_counter() - t0)
+        with open(a, "rb") as fa, open(b, "rb") as fb:
+            same = fa.read() == fb.read()
+    return {"device_pack_s": round(min(td), 4), "download_and_host_writer_s": round(min(th), 4),
+            "identical": same}
+
+
 def cylinder_bench(steps=50000):
     """Channel past a cylinder, 2048 x 512, Re = 100: FP64 and FP32 storage with the
     FP64 collision, `steps` steps each with the drag/lift history recorded on the
@@ -337,6 +360,8 @@ def cylinder_bench(steps=50000):
         lat.synchronize()
         wall = time.perf_counter() - t0
         rec[name] = lat.force_record()
+        if name == "fp64":
+            out["vtk_output"] = vtk_output_times(lat)
         lat.close()
         out[name] = {"MLUPS": round(2048 * 512 * steps / wall / 1e6, 1),
                      "ms_per_step": round(wall / steps * 1e3, 4)}

@@ -196,6 +196,14 @@ LBMX_API int lbmx_moments(lbmx_lattice *lat, double *rho, double *u, double *v);
  * second one is completed on the device by this call. */
 LBMX_API int lbmx_download(lbmx_lattice *lat, double *f, double *f_new);
 
+/* write_vtk_binary_2D (lbm_utils.h; write_vtk_binary_2D.c:47-78) of the lattice's
+ * current rho/u/v -- the moments lbmx_moments would return -- with the velocity
+ * scaled by cu and the density by crho.  The big-endian float32 blocks are packed
+ * on the device, so only 16 B per node cross PCIe; the file is byte-identical to
+ * the host writer's. */
+LBMX_API int lbmx_write_vtk(lbmx_lattice *lat, const char *filename, double dx, double cu,
+                            double crho);
+
 /* Blocks until the lattice's queued work is done. */
 LBMX_API int lbmx_synchronize(lbmx_lattice *lat);
 

@@ -41,6 +41,22 @@ static uint32_t to_big_endian_bits(float x)
     return b;
 }
 
+/* The text parts of the VTK file, shared with the device-packed writer
+ * (lbmx_write_vtk): everything before the velocity block, and between the
+ * velocity and density blocks. */
+__attribute__((visibility("hidden"))) void lbm_vtk_head(FILE *fd, int Nx, int Ny, double dx)
+{
+    fprintf(fd, "# vtk DataFile Version 3.0\n2D LBM output\nBINARY\n");
+    fprintf(fd, "DATASET STRUCTURED_POINTS\nDIMENSIONS %d %d 1\nORIGIN 0 0 0\n", Nx, Ny);
+    fprintf(fd, "SPACING %f %f 1\n\nPOINT_DATA %d\n", dx, dx, Nx * Ny);
+    fprintf(fd, "VECTORS velocity float\n");
+}
+
+__attribute__((visibility("hidden"))) void lbm_vtk_mid(FILE *fd)
+{
+    fprintf(fd, "\nSCALARS density float 1\nLOOKUP_TABLE default\n");
+}
+
 /* Legacy VTK STRUCTURED_POINTS file with big-endian float32 VECTORS velocity and
  * SCALARS density, the layout src/utils/write_vtk_binary_2D.c:46-78 produces.
  * Unlike the reference this version keeps no static buffer (re-entrant). */
@@ -52,9 +68,7 @@ void write_vtk_binary_2D(const char *filename, int Nx, int Ny, double dx, double
         perror("Failed to open VTK file");
         return;
     }
-    fprintf(fd, "# vtk DataFile Version 3.0\n2D LBM output\nBINARY\n");
-    fprintf(fd, "DATASET STRUCTURED_POINTS\nDIMENSIONS %d %d 1\nORIGIN 0 0 0\n", Nx, Ny);
-    fprintf(fd, "SPACING %f %f 1\n\nPOINT_DATA %d\n", dx, dx, Nx * Ny);
+    lbm_vtk_head(fd, Nx, Ny, dx);
 
     const size_t nn = (size_t)Nx * (size_t)Ny;
     uint32_t *vel = malloc(nn * 3 * sizeof(uint32_t));
@@ -76,9 +90,8 @@ void write_vtk_binary_2D(const char *filename, int Nx, int Ny, double dx, double
             den[n] = to_big_endian_bits((float)(rho[j][i] * crho));
         }
     }
-    fprintf(fd, "VECTORS velocity float\n");
     fwrite(vel, sizeof(uint32_t), nn * 3, fd);
-    fprintf(fd, "\nSCALARS density float 1\nLOOKUP_TABLE default\n");
+    lbm_vtk_mid(fd);
     fwrite(den, sizeof(uint32_t), nn, fd);
     free(vel);
     free(den);

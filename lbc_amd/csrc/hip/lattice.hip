@@ -258,6 +258,8 @@ extern "C" void lbmx_destroy(lbmx_lattice *L)
     for (double *p : L->val1) hipFree(p);
     for (double *p : L->val2) hipFree(p);
     hipFree(L->staging);
+    hipFree(L->vtk_dev);
+    if (L->vtk_host) hipHostFree(L->vtk_host);
     hipFree(L->tdibc.st);
     hipFree(L->tdibc.last);
     hipFree(L->tdibc.active);
@@ -1499,6 +1501,95 @@ extern "C" int lbmx_moments(lbmx_lattice *L, double *rho, double *u, double *v)
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(L->stream));
     return LBMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// VTK output packed on the device (write_vtk_binary_2D.c:47-78 layout)
+// ---------------------------------------------------------------------------
+extern "C" void lbm_vtk_head(FILE *fd, int Nx, int Ny, double dx);
+extern "C" void lbm_vtk_mid(FILE *fd);
+
+// (float)(x * c) as big-endian bits.  A NaN (solid nodes) converts as the x86
+// host writer converts it: sign and the top mantissa bits kept, quiet bit set.
+__device__ __forceinline__ uint32_t vtk_be(double x, double c)
+{
+    uint32_t b;
+    if (isnan(x)) {
+        const unsigned long long d = (unsigned long long)__double_as_longlong(x);
+        b = (uint32_t)((d >> 32) & 0x80000000ull) | 0x7fc00000u | (uint32_t)((d >> 29) & 0x3fffffull);
+    } else {
+        b = __float_as_uint((float)(x * c));
+    }
+    return __builtin_bswap32(b);
+}
+
+__global__ void __launch_bounds__(256) k_vtk_pack(const double *__restrict__ R,
+                                                  const double *__restrict__ U,
+                                                  const double *__restrict__ V, int Nx,
+                                                  int pitch, long long nn, double cu,
+                                                  double crho, uint32_t *__restrict__ out)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= Nx) return;
+    const size_t s = (size_t)y * pitch + x;
+    const size_t n = (size_t)y * Nx + x;
+    out[3 * n + 0] = vtk_be(U[s], cu);
+    out[3 * n + 1] = vtk_be(V[s], cu);
+    out[3 * n + 2] = 0u;   // big-endian 0.0f
+    out[3 * nn + n] = vtk_be(R[s], crho);
+}
+
+extern "C" int lbmx_write_vtk(lbmx_lattice *L, const char *filename, double dx, double cu,
+                              double crho)
+{
+    if (!L || !filename) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!L->has_state) return lbmx_fail(LBMX_ERR_STATE, "no state uploaded");
+    if (int prc = lbmx_ensure_push(L)) return prc;
+    const double *r = L->R, *u = L->U, *v = L->V;
+    if (L->steps > 0) {
+        int rc = lbmx_launch_moments(L);
+        if (rc) return rc;
+        r = L->Ro;
+        u = L->Uo;
+        v = L->Vo;
+    }
+    const long long nn = (long long)L->Nx * L->Ny;
+    const size_t chunk = (size_t)32 << 20;   // bytes per pinned host transfer
+    if (!L->vtk_dev) {
+        if (hipMalloc((void **)&L->vtk_dev, (size_t)nn * 16) != hipSuccess)
+            return lbmx_fail(LBMX_ERR_NOMEM, "VTK pack buffer (%lld bytes)", nn * 16);
+        L->bytes += nn * 16;
+    }
+    if (!L->vtk_host) HIPCHK(hipHostMalloc((void **)&L->vtk_host, chunk));
+    dim3 grid((L->Nx + 255) / 256, L->Ny);
+    hipLaunchKernelGGL(k_vtk_pack, grid, dim3(256), 0, L->stream, r, u, v, L->Nx, L->pitch, nn,
+                       cu, crho, L->vtk_dev);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(L->stream));
+
+    FILE *fd = fopen(filename, "wb");
+    if (!fd) return lbmx_fail(LBMX_ERR_ARG, "cannot open %s for writing", filename);
+    int rc = LBMX_OK;
+    // velocity block (12 B per node), the text between, the density block (4 B)
+    auto emit = [&](const uint32_t *dev, size_t bytes) -> int {
+        for (size_t off = 0; off < bytes; off += chunk) {
+            const size_t len = std::min(chunk, bytes - off);
+            HIPCHK(hipMemcpyAsync(L->vtk_host, (const char *)dev + off, len,
+                                  hipMemcpyDeviceToHost, L->stream));
+            HIPCHK(hipStreamSynchronize(L->stream));
+            if (fwrite(L->vtk_host, 1, len, fd) != len)
+                return lbmx_fail(LBMX_ERR_ARG, "short write to %s", filename);
+        }
+        return LBMX_OK;
+    };
+    lbm_vtk_head(fd, L->Nx, L->Ny, dx);
+    rc = emit(L->vtk_dev, (size_t)nn * 12);
+    if (!rc) {
+        lbm_vtk_mid(fd);
+        rc = emit(L->vtk_dev + 3 * nn, (size_t)nn * 4);
+    }
+    if (fclose(fd) != 0 && !rc) rc = lbmx_fail(LBMX_ERR_ARG, "cannot close %s", filename);
+    return rc;
 }
 
 extern "C" int lbmx_synchronize(lbmx_lattice *L)

@@ -107,6 +107,10 @@ struct lbmx_lattice {
     int64_t bytes = 0;
     double *staging = nullptr;
     size_t staging_bytes = 0;
+    // lbmx_write_vtk: packed big-endian float32 output (16 B per node) on the
+    // device and a pinned host chunk it is copied through
+    uint32_t *vtk_dev = nullptr;
+    uint32_t *vtk_host = nullptr;
 };
 
 lbmx::Geo lbmx_geo(const lbmx_lattice *L);

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -103,6 +104,9 @@ def _bind():
     L.lbmx_device_bytes.restype = C.c_int64
     L.lbmx_tail_plan.argtypes = [i, i, i, i, _ip, _ip, _ip, i, _ip, _ip]
     L.lbmx_copy_rate.argtypes = [d, _dp]
+    L.lbmx_write_vtk.argtypes = [vp, C.c_char_p, d, d, d]
+    L.write_vtk_binary_2D.argtypes = [C.c_char_p, i, i, d, _dp, _dp, _dp, d, d]
+    L.write_vtk_binary_2D.restype = None
     _bound = True
     return L
 
@@ -134,6 +138,14 @@ def mei_initialization(f, f_new, rho_in, U_in, V_in, rho, u, v, omega, per_x, pe
     if n < 0:
         check(n)
     return n
+
+
+def write_vtk_host(filename, dx, u, v, rho, cu=1.0, crho=1.0):
+    """The drop-in host writer write_vtk_binary_2D (lbm_utils.h) on numpy arrays."""
+    Ny, Nx = rho.shape
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, rho)]
+    _bind().write_vtk_binary_2D(os.fsencode(filename), Nx, Ny, float(dx), *(_d(a) for a in arrs),
+                                float(cu), float(crho))
 
 
 def copy_rate(gib=8.0):
@@ -287,6 +299,11 @@ class Lattice:
         v = np.empty((self.Ny, self.Nx))
         check(self._L.lbmx_moments(self.h, _d(rho), _d(u), _d(v)))
         return rho, u, v
+
+    def write_vtk(self, filename, dx=1.0, cu=1.0, crho=1.0):
+        """write_vtk_binary_2D of the current moments, packed on the device."""
+        check(self._L.lbmx_write_vtk(self.h, os.fsencode(filename), float(dx), float(cu),
+                                     float(crho)))
 
     def download(self, both=False):
         f = np.empty((self.Ny, self.Nx, 9))
