@@ -142,6 +142,7 @@ def cylinder_lattice(precision):
     phi = np.where((j == 0) | (j == Ny - 1), -1e30, d)
     lat = native.Lattice(Nx, Ny, precision=precision, per_x=0, per_y=0, use_ibb=1)
     lat.set_geometry(mask, np.full((Ny, Nx), 1.0 / tau), phi)
+    lat.set_force_body(d <= 0)          # the cylinder alone, besides the walls-included total
     lat.set_boundaries([("velocity_inlet_regularized", 0),
                         ("pressure_outlet_regularized", Nx - 1)])
     y = np.arange(Ny) - 0.5
@@ -345,9 +346,10 @@ def vtk_output_times(lat, reps=3):
 def cylinder_bench(steps=50000):
     """Channel past a cylinder, 2048 x 512, Re = 100: FP64 and FP32 storage with the
     FP64 collision, `steps` steps each with the drag/lift history recorded on the
-    device every step (the cylinder driver's {t, L, D} record).  Reported: MLUPS
-    and the FP32 history's largest deviation from FP64, relative to the FP64
-    run's largest |D| and |L|."""
+    device every step (the cylinder driver's {t, L, D} record, walls included, and
+    the cylinder alone beside it).  Reported: MLUPS, the FP32 history's largest
+    deviation from FP64, relative to the FP64 run's largest |D| and |L|, the last
+    step's forces, and one VTK output step packed on the device vs on the host."""
     from lbc_amd import native
     out = {"workload": f"cylinder_channel_2048x512_re100_ibb_regularized_{steps}_steps"}
     rec = {}
@@ -361,6 +363,8 @@ def cylinder_bench(steps=50000):
         wall = time.perf_counter() - t0
         rec[name] = lat.force_record()
         if name == "fp64":
+            body = lat.body_force_record()
+            out["cylinder_only_last"] = {"drag": float(body[-1, 2]), "lift": float(body[-1, 1])}
             out["vtk_output"] = vtk_output_times(lat)
         lat.close()
         out[name] = {"MLUPS": round(2048 * 512 * steps / wall / 1e6, 1),

@@ -173,8 +173,6 @@ LBMX_API int lbmx_probe_history(lbmx_lattice *lat, double *rec, int rows);
  * *ms receives the device time between them (blocks until done). */
 LBMX_API int lbmx_time_steps(lbmx_lattice *lat, int nsteps, double *ms);
 
-/* Momentum-exchange force of the last step, as main_lbm returns it through
- * (L, D) (main_lbm.c:43): D = x-component, L = y-component. */
 /* Force record on the device: from the next step on, every step appends the row
  * {t = row*dt, L*cf, D*cf} -- the record the cylinder driver writes per step
  * (cases/cylinder/cylinder.c:162-185) -- without a host read per step.
@@ -183,7 +181,21 @@ LBMX_API int lbmx_set_force_record(lbmx_lattice *lat, double dt, double cf, int 
 /* Copies up to capacity recorded rows (3 doubles each); returns the row count. */
 LBMX_API int lbmx_force_record(lbmx_lattice *lat, double *rows, int capacity);
 
+/* Momentum-exchange force of the last step, as main_lbm returns it through
+ * (L, D) (main_lbm.c:43): D = x-component, L = y-component, summed over every
+ * fluid->solid link, walls included (compute_force_mem.c:26-58). */
 LBMX_API int lbmx_forces(lbmx_lattice *lat, double *L, double *D);
+
+/* Force on one body (an extension: the reference has only the total above, which
+ * includes the channel walls).  body is Ny x Nx, nonzero on the solid nodes of the
+ * body, or NULL for none; links into those nodes are summed separately from the
+ * next step on, with no change to the total.  lbmx_set_geometry clears it.
+ * lbmx_body_forces returns the last step's body force like lbmx_forces;
+ * lbmx_body_force_record the rows {t, L_body*cf, D_body*cf} recorded alongside
+ * lbmx_force_record's. */
+LBMX_API int lbmx_set_force_body(lbmx_lattice *lat, const int *body);
+LBMX_API int lbmx_body_forces(lbmx_lattice *lat, double *L, double *D);
+LBMX_API int lbmx_body_force_record(lbmx_lattice *lat, double *rows, int capacity);
 
 /* Moments of the current state exactly as compute_macroscopic_fields leaves
  * them after the last step (NaN on solid nodes).  Any pointer may be NULL. */

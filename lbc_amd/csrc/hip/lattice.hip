@@ -212,8 +212,8 @@ extern "C" int lbmx_create(const lbmx_config *cfg, lbmx_lattice **out)
         hipMalloc((void **)&L->R, fld_bytes) != hipSuccess ||
         hipMalloc((void **)&L->U, fld_bytes) != hipSuccess ||
         hipMalloc((void **)&L->V, fld_bytes) != hipSuccess ||
-        hipMalloc((void **)&L->force_dev, 2 * sizeof(double)) != hipSuccess ||
-        hipHostMalloc((void **)&L->force_host, 2 * sizeof(double), hipHostMallocDefault) !=
+        hipMalloc((void **)&L->force_dev, kPartial * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void **)&L->force_host, kPartial * sizeof(double), hipHostMallocDefault) !=
             hipSuccess) {
         lbmx_destroy(L);
         return lbmx_fail(LBMX_ERR_NOMEM, "device allocation of %zu bytes failed",
@@ -224,7 +224,7 @@ extern "C" int lbmx_create(const lbmx_config *cfg, lbmx_lattice **out)
     // every byte of the buffers is defined
     hipMemsetAsync(L->pop[0], 0, pop_bytes, L->stream);
     hipMemsetAsync(L->pop[1], 0, pop_bytes, L->stream);
-    hipMemsetAsync(L->force_dev, 0, 2 * sizeof(double), L->stream);
+    hipMemsetAsync(L->force_dev, 0, kPartial * sizeof(double), L->stream);
     // default geometry: all fluid, omega = 1, no boundaries
     L->geom = 0;
     L->om_uniform = 1.0;
@@ -371,6 +371,7 @@ extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double 
     // fluid -> solid links, row-major, k ascending (compute_force_mem.c:26-58);
     // q and the behind node as interpolated_bounce_back.c:51-89 derives them
     std::vector<Link> links;
+    std::vector<long long> target;   // the link's solid node, y * Nx + x
     const double eps = 1e-12;
     const int px = L->cfg.per_x, py = L->cfg.per_y;
     if (nsolid) {
@@ -387,6 +388,8 @@ extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double 
                     l.q = 0.5;
                     l.nb = -1;
                     l.bsolid = 0;
+                    l.body = 0;
+                    l.pad_ = 0;
                     if (L->cfg.use_ibb) {
                         double den = phi[(size_t)j * Nx + i] - phi[(size_t)jn * Nx + in];
                         if (fabs(den) < eps) den = (den >= 0.0) ? eps : -eps;
@@ -401,6 +404,7 @@ extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double 
                         }
                     }
                     links.push_back(l);
+                    target.push_back((long long)jn * Nx + in);
                 }
             }
     }
@@ -412,9 +416,11 @@ extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double 
         HIPCHK(hipMalloc((void **)&L->links, links.size() * sizeof(Link)));
         HIPCHK(hipMemcpy(L->links, links.data(), links.size() * sizeof(Link),
                          hipMemcpyHostToDevice));
-        HIPCHK(hipMalloc((void **)&L->partials, 2 * sizeof(double) * L->npartials));
+        HIPCHK(hipMalloc((void **)&L->partials, kPartial * sizeof(double) * L->npartials));
     }
-    HIPCHK(hipMemsetAsync(L->force_dev, 0, 2 * sizeof(double), L->stream));
+    L->hlinks = std::move(links);
+    L->link_target = std::move(target);
+    HIPCHK(hipMemsetAsync(L->force_dev, 0, kPartial * sizeof(double), L->stream));
 
     // stale-edge enumeration (k_edges)
     L->edge_rows = py ? 0 : 2 * Nx;
@@ -441,7 +447,9 @@ extern "C" int lbmx_set_uniform(lbmx_lattice *L, double omega)
     L->tail_planned = 0;
     L->nlinks = 0;
     L->npartials = 0;
-    HIPCHK(hipMemsetAsync(L->force_dev, 0, 2 * sizeof(double), L->stream));
+    L->hlinks.clear();
+    L->link_target.clear();
+    HIPCHK(hipMemsetAsync(L->force_dev, 0, kPartial * sizeof(double), L->stream));
     L->edge_rows = L->cfg.per_y ? 0 : 2 * L->Nx;
     L->edge_cols = L->cfg.per_x ? 0 : (L->cfg.per_y ? 2 * L->Ny : 2 * (L->Ny - 2));
     L->has_geometry = 1;
@@ -1414,11 +1422,36 @@ extern "C" int lbmx_time_steps(lbmx_lattice *L, int nsteps, double *ms)
 extern "C" int lbmx_forces(lbmx_lattice *L, double *Lift, double *Drag)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
-    HIPCHK(hipMemcpyAsync(L->force_host, L->force_dev, 2 * sizeof(double),
+    HIPCHK(hipMemcpyAsync(L->force_host, L->force_dev, kPartial * sizeof(double),
                           hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
     if (Drag) *Drag = L->force_host[0];
     if (Lift) *Lift = L->force_host[1];
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_body_forces(lbmx_lattice *L, double *Lift, double *Drag)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    HIPCHK(hipMemcpyAsync(L->force_host, L->force_dev, kPartial * sizeof(double),
+                          hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    if (Drag) *Drag = L->force_host[2];
+    if (Lift) *Lift = L->force_host[3];
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_set_force_body(lbmx_lattice *L, const int *body)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!L->has_geometry) return lbmx_fail(LBMX_ERR_STATE, "lbmx_set_geometry first");
+    if (int prc = lbmx_ensure_push(L)) return prc;
+    if (L->nlinks == 0) return LBMX_OK;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    for (size_t l = 0; l < L->hlinks.size(); l++)
+        L->hlinks[l].body = (body && body[L->link_target[l]]) ? 1 : 0;
+    HIPCHK(hipMemcpy(L->links, L->hlinks.data(), L->hlinks.size() * sizeof(Link),
+                     hipMemcpyHostToDevice));
     return LBMX_OK;
 }
 
@@ -1432,7 +1465,8 @@ extern "C" int lbmx_set_force_record(lbmx_lattice *L, double dt, double cf, int 
     L->frec_cap = 0;
     L->frec_row = 0;
     if (capacity == 0) return LBMX_OK;
-    if (hipMalloc((void **)&L->frec, (size_t)capacity * 3 * sizeof(double)) != hipSuccess) {
+    if (hipMalloc((void **)&L->frec, (size_t)capacity * kRecordCols * sizeof(double)) !=
+        hipSuccess) {
         L->frec = nullptr;
         return lbmx_fail(LBMX_ERR_NOMEM, "force record of %d rows", capacity);
     }
@@ -1449,7 +1483,29 @@ extern "C" int lbmx_force_record(lbmx_lattice *L, double *rows, int capacity)
     long long n = L->frec_row < L->frec_cap ? L->frec_row : L->frec_cap;
     if (n > capacity) n = capacity;
     HIPCHK(hipStreamSynchronize(L->stream));
-    if (n > 0) HIPCHK(hipMemcpy(rows, L->frec, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (n > 0)
+        HIPCHK(hipMemcpy2D(rows, 3 * sizeof(double), L->frec, kRecordCols * sizeof(double),
+                           3 * sizeof(double), (size_t)n, hipMemcpyDeviceToHost));
+    return (int)n;
+}
+
+extern "C" int lbmx_body_force_record(lbmx_lattice *L, double *rows, int capacity)
+{
+    if (!L || !rows || capacity < 0) return lbmx_fail(LBMX_ERR_ARG, "invalid argument");
+    if (!L->frec) return 0;
+    long long n = L->frec_row < L->frec_cap ? L->frec_row : L->frec_cap;
+    if (n > capacity) n = capacity;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    if (n > 0) {
+        std::vector<double> all((size_t)n * kRecordCols);
+        HIPCHK(hipMemcpy(all.data(), L->frec, all.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+        for (long long r = 0; r < n; r++) {
+            rows[3 * r + 0] = all[kRecordCols * r + 0];
+            rows[3 * r + 1] = all[kRecordCols * r + 3];
+            rows[3 * r + 2] = all[kRecordCols * r + 4];
+        }
+    }
     return (int)n;
 }
 

@@ -44,6 +44,8 @@ struct lbmx_lattice {
     long long nsolid = 0;
     lbmx::Link *links = nullptr;
     int nlinks = 0;
+    std::vector<lbmx::Link> hlinks;          // host copy of the link list
+    std::vector<long long> link_target;      // each link's solid node, y * Nx + x
     double *partials = nullptr;
     int npartials = 0;
     double *force_dev = nullptr;

@@ -438,9 +438,12 @@ struct Link {
     double q;       // wall distance fraction, clamped (interpolated_bounce_back.c:51-61)
     int k;          // direction pointing into the solid
     int bsolid;     // behind node is solid
+    int body;       // the solid node x + c_k belongs to the force body (lbmx_set_force_body)
+    int pad_;
 };
 
 constexpr int kLinkThreads = 256;
+constexpr int kPartial = 4;   // doubles per link block: total (x, y), body (x, y)
 
 struct LinkArgs {
     Geo g;
@@ -451,31 +454,29 @@ struct LinkArgs {
     int ibb;
     int write_bb;       // also store plain bounce-back values (stand-alone bounce_back)
     int read_val;       // force only: take f_new(x, opp k) from D instead of computing it
-    double *partials;   // 2 doubles per block
+    double *partials;   // kPartial doubles per block
 };
 
-__device__ __forceinline__ void block_sum2(double &fx, double &fy, double *lds)
+// Fixed butterfly within each wavefront, then the waves summed in order; thread 0
+// of the block ends with the block sums.
+__device__ __forceinline__ void block_sum(double v[kPartial], double *lds)
 {
-    // fixed butterfly within each wavefront, then waves summed in order
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        fx += __shfl_xor(fx, o, 64);
-        fy += __shfl_xor(fy, o, 64);
-    }
+    for (int o = 32; o >= 1; o >>= 1)
+#pragma unroll
+        for (int c = 0; c < kPartial; c++) v[c] += __shfl_xor(v[c], o, 64);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    if (lane == 0) {
-        lds[2 * wave] = fx;
-        lds[2 * wave + 1] = fy;
-    }
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < kPartial; c++) lds[kPartial * wave + c] = v[c];
     __syncthreads();
     if (threadIdx.x == 0) {
-        double sx = 0.0, sy = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / 64); w++) {
-            sx += lds[2 * w];
-            sy += lds[2 * w + 1];
-        }
-        fx = sx;
-        fy = sy;
+        double s[kPartial] = {0.0, 0.0, 0.0, 0.0};
+        for (int w = 0; w < (int)(blockDim.x / 64); w++)
+#pragma unroll
+            for (int c = 0; c < kPartial; c++) s[c] += lds[kPartial * w + c];
+#pragma unroll
+        for (int c = 0; c < kPartial; c++) v[c] = s[c];
     }
 }
 
@@ -508,9 +509,9 @@ template <typename T>
 __device__ __forceinline__ void links_block(const T *__restrict__ S, T *__restrict__ D,
                                             const LinkArgs &a, int blk)
 {
-    __shared__ double lds[2 * kLinkThreads / 64];
+    __shared__ double lds[kPartial * kLinkThreads / 64];
     const int l = blk * kLinkThreads + threadIdx.x;
-    double fx = 0.0, fy = 0.0;
+    double f[kPartial] = {0.0, 0.0, 0.0, 0.0};
     if (l < a.nlinks) {
         const Link L = a.links[l];
         const int k = L.k, ko = c_opp[k];
@@ -530,14 +531,17 @@ __device__ __forceinline__ void links_block(const T *__restrict__ S, T *__restri
             pop_put(D, (size_t)ko * a.g.plane + (size_t)L.n, ko, val, a.g.rho0);
         // compute_force_mem.c:55-57: f*(x,k) + f_new(x,opp k) after (I)BB
         const double df = fk + val;
-        fx = df * (double)c_cx[k];
-        fy = df * (double)c_cy[k];
+        f[0] = df * (double)c_cx[k];
+        f[1] = df * (double)c_cy[k];
+        if (L.body) {
+            f[2] = f[0];
+            f[3] = f[1];
+        }
     }
-    block_sum2(fx, fy, lds);
-    if (threadIdx.x == 0) {
-        a.partials[2 * blk] = fx;
-        a.partials[2 * blk + 1] = fy;
-    }
+    block_sum(f, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < kPartial; c++) a.partials[kPartial * blk + c] = f[c];
 }
 
 template <typename T>
@@ -586,8 +590,9 @@ struct BcArgs {
     int nbc;
     const double *partials;
     int npartials;
-    double *force_out;   // [0] = x (the reference's D), [1] = y (L)
-    double *record;      // optional {t, L, D} rows (cases/cylinder/cylinder.c:162-185)
+    double *force_out;   // [0] = x (the reference's D), [1] = y (L); [2], [3] the body's
+    double *record;      // optional rows {t, L, D} (cases/cylinder/cylinder.c:162-185)
+                         // followed by the body's {L, D}
     long long row;       // row of this step
     int capacity;
     double dt, cf;       // t = row * dt, L and D scaled by cf
@@ -834,24 +839,28 @@ __device__ __forceinline__ void bc_apply(const T *S, T *D, const BcArgs &a, cons
     }
 }
 
-// main_lbm's force outputs (fixed-order sum of the link partials) and the record
+// main_lbm's force outputs (fixed-order sum of the link partials; force_out[0..1]
+// all links as the reference sums them, [2..3] the links into the force body) and
+// the record row {t, L, D, L_body, D_body}
+constexpr int kRecordCols = 5;
+
 __device__ __forceinline__ void bc_force_total(const BcArgs &a)
 {
+    double s[kPartial] = {0.0, 0.0, 0.0, 0.0};
     if (a.npartials > 0) {
-        double fx = 0.0, fy = 0.0;
-        for (int p = 0; p < a.npartials; p++) {
-            fx += a.partials[2 * p];
-            fy += a.partials[2 * p + 1];
-        }
-        a.force_out[0] = fx;
-        a.force_out[1] = fy;
+        for (int p = 0; p < a.npartials; p++)
+#pragma unroll
+            for (int c = 0; c < kPartial; c++) s[c] += a.partials[kPartial * p + c];
+#pragma unroll
+        for (int c = 0; c < kPartial; c++) a.force_out[c] = s[c];
     }
     if (a.record && a.row < a.capacity) {
-        const double fx = a.npartials > 0 ? a.force_out[0] : 0.0;
-        const double fy = a.npartials > 0 ? a.force_out[1] : 0.0;
-        a.record[3 * a.row + 0] = (double)a.row * a.dt;
-        a.record[3 * a.row + 1] = fy * a.cf;
-        a.record[3 * a.row + 2] = fx * a.cf;
+        double *r = a.record + (size_t)kRecordCols * a.row;
+        r[0] = (double)a.row * a.dt;
+        r[1] = s[1] * a.cf;
+        r[2] = s[0] * a.cf;
+        r[3] = s[3] * a.cf;
+        r[4] = s[2] * a.cf;
     }
 }
 

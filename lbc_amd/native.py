@@ -95,6 +95,9 @@ def _bind():
     L.lbmx_forces.argtypes = [vp, _dp, _dp]
     L.lbmx_set_force_record.argtypes = [vp, d, d, i]
     L.lbmx_force_record.argtypes = [vp, _dp, i]
+    L.lbmx_set_force_body.argtypes = [vp, _ip]
+    L.lbmx_body_forces.argtypes = [vp, _dp, _dp]
+    L.lbmx_body_force_record.argtypes = [vp, _dp, i]
     L.lbmx_moments.argtypes = [vp, _dp, _dp, _dp]
     L.lbmx_download.argtypes = [vp, _dp, _dp]
     L.lbmx_synchronize.argtypes = [vp]
@@ -292,6 +295,24 @@ class Lattice:
         Lf, Df = C.c_double(), C.c_double()
         check(self._L.lbmx_forces(self.h, C.byref(Lf), C.byref(Df)))
         return Lf.value, Df.value
+
+    def set_force_body(self, body):
+        """Solid nodes (nonzero in the Ny x Nx array) whose links form the body
+        force; None clears it."""
+        b = None if body is None else np.ascontiguousarray(body, dtype=np.int32)
+        check(self._L.lbmx_set_force_body(self.h, _i(b)))
+
+    def body_forces(self):
+        Lf, Df = C.c_double(), C.c_double()
+        check(self._L.lbmx_body_forces(self.h, C.byref(Lf), C.byref(Df)))
+        return Lf.value, Df.value
+
+    def body_force_record(self):
+        rows = np.zeros((max(1, self._frec_cap), 3))
+        n = self._L.lbmx_body_force_record(self.h, _d(rows), rows.shape[0])
+        if n < 0:
+            check(n)
+        return rows[:n]
 
     def moments(self):
         rho = np.empty((self.Ny, self.Nx))

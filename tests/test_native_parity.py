@@ -271,3 +271,67 @@ def test_tgv256_analytic_decay():
     err = np.sqrt((((u - ua) ** 2 + (v - va) ** 2).sum()) / ((ua ** 2 + va ** 2).sum()))
     print(f"TGV 256^2, {T} steps: L2 deviation from the analytic vortex {err:.3e}")
     assert err < 3e-3
+
+
+def oracle_body_force(lat0, sc, body):
+    """compute_force_mem over the links into `body` only, in the reference's step
+    order up to the force (collision in place, then (interpolated) bounce-back
+    into f_new; main_lbm.c:31-43) on copies of the oracle lattice.  With
+    body = the full solid mask this is the step's own (L, D)."""
+    import ctypes as C
+    Nx, Ny = lat0.Nx, lat0.Ny
+    f, fn = lat0.f.copy(), lat0.f_new.copy()
+    p = O._p
+    O.ORACLE.orc_collision(Nx, Ny, p(f), p(lat0.rho), p(lat0.u), p(lat0.v), p(lat0.omega),
+                           p(lat0.mask), sc.Fx, sc.Fy)
+    if sc.use_ibb:
+        O.ORACLE.orc_interpolated_bounce_back(Nx, Ny, p(f), p(fn), p(lat0.mask), p(lat0.phi),
+                                              sc.per_x, sc.per_y)
+    else:
+        O.ORACLE.orc_bounce_back(Nx, Ny, p(f), p(fn), p(lat0.mask), sc.per_x, sc.per_y)
+    fx, fy = C.c_double(), C.c_double()
+    O.ORACLE.orc_force_mem(Nx, Ny, p(f), p(fn), p(np.ascontiguousarray(body, dtype=np.int32)),
+                           sc.per_x, sc.per_y, C.byref(fx), C.byref(fy))
+    return fy.value, fx.value
+
+
+BODY_CASES = [s for s in all_scenarios(2) if s.name in ("channel_regularized_ibb", "cylinder_ibb")]
+
+
+@pytest.mark.parametrize("sc", BODY_CASES, ids=[s.name for s in BODY_CASES])
+def test_body_force_record(sc):
+    """Drag and lift of the disk alone (walls excluded) per step, recorded on the
+    device, against the oracle's momentum exchange over the disk's links; the
+    total (walls included) is unchanged."""
+    from lbc_amd import native
+    body = (sc.mask != 0) & (sc.phi > -1e29)          # the disk, not the wall rows
+    assert body.any()
+    state = sc.initial_state()
+    lat0 = oracle_lattice(sc, state)
+    lat = native.Lattice(sc.Nx, sc.Ny, precision=0, per_x=sc.per_x, per_y=sc.per_y,
+                         use_ibb=sc.use_ibb, Fx=sc.Fx, Fy=sc.Fy)
+    lat.set_geometry(sc.mask, sc.omega, sc.phi)
+    lat.set_boundaries(sc.bc_names)
+    lat.upload(*state)
+    lat.set_force_body(body)
+    lat.set_force_record(NSTEPS, dt=0.5, cf=2.0)
+    ref_body, ref_total = [], []
+    for step in range(NSTEPS):
+        vals = sc.vals(step)
+        ref_body.append(oracle_body_force(lat0, sc, body))
+        check_total = oracle_body_force(lat0, sc, sc.mask)
+        ref_total.append(O.oracle_step(lat0, vals, sc.per_x, sc.per_y, sc.Fx, sc.Fy, sc.use_ibb))
+        assert check_total == ref_total[-1]        # the procedure is the step's own
+        for b, (_, _, v1, v2) in enumerate(vals):
+            lat.set_bc_values(b, v1, v2)
+        lat.step(1)
+    rec, brec = lat.force_record(), lat.body_force_record()
+    Lb, Db = lat.body_forces()
+    lat.close()
+    ref_body, ref_total = np.array(ref_body), np.array(ref_total)
+    assert np.array_equal(brec[:, 0], rec[:, 0])
+    np.testing.assert_allclose(rec[:, 1:], ref_total * 2.0, rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(brec[:, 1:], ref_body * 2.0, rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose([Lb, Db], ref_body[-1], rtol=1e-11, atol=1e-12)
+    if (sc.phi < -1e29).any():                         # the walls do carry force
+        assert np.abs(ref_body - ref_total).max() > 1e-6
