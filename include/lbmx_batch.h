@@ -90,14 +90,17 @@ typedef struct {
     int keep_moments;          /* store rho/u/v when the solve completes */
 } lbmx_case;
 
-/* Launch configuration of the persistent interior kernel (chosen by lbmx_batch_tune). */
+/* Launch configuration of the interior kernel (chosen by lbmx_batch_tune; default
+ * {256, 512, 16, 5}). */
 typedef struct {
     int threads;               /* threads per workgroup: 64, 128 or 256 */
     int tile;                  /* nodes per tile: threads * {1 .. 16}, >= 256 */
-    int groups_per_cu;         /* persistent workgroups per compute unit */
-    int order;                 /* tiles to workgroups: 0 interleaved (tile t on workgroup
-                                  t mod G), 1 one contiguous band per XCD, 2 one contiguous
-                                  range per workgroup */
+    int groups_per_cu;         /* persistent workgroups per compute unit (orders 0-4) */
+    int order;                 /* tiles to workgroups: 5 one tile per workgroup (grid
+                                  covering the table); persistent grids: 0 interleaved
+                                  (tile t on workgroup t mod G), 1 one contiguous band per
+                                  XCD, 2 one contiguous range per workgroup, 3 / 4 bands of
+                                  4 / 16 tiles dealt to one XCD each */
 } lbmx_tuning;
 
 typedef struct lbmx_batch lbmx_batch;

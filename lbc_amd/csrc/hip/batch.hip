@@ -327,8 +327,19 @@ static int upload_tables(lbmx_batch *b, BatchSlot &S)
 // ---- launches ---------------------------------------------------------------------------
 static int grid_of(const lbmx_batch *b)
 {
+    if (b->tun.order == 5) return std::max(8, b->grid5);
     int G = b->ncu * b->tun.groups_per_cu;
     return std::max(8, (G / 8) * 8);
+}
+
+// Interior grid of tile order 5 for n tiles: n rounded up to a multiple of the
+// largest power of two <= n / 8 (at most 1/8 idle workgroups), so the grid -- and
+// with it the captured graphs -- changes only when n crosses such a step.
+static int grid5_of(int n)
+{
+    int q = 8;
+    while (q * 16 <= n) q *= 2;
+    return std::max(q, (n + q - 1) / q * q);
 }
 
 template <typename T, bool W>
@@ -378,10 +389,20 @@ static int launch_init(lbmx_batch *b, int slot)
     return LBMX_OK;
 }
 
+// Destroys the retired graphs; the stream must be idle.
+static void reap_graphs(lbmx_batch *b)
+{
+    for (hipGraphExec_t g : b->retired) hipGraphExecDestroy(g);
+    b->retired.clear();
+}
+
+// Retires the captured graphs (a launch of them may still be queued); they are
+// destroyed at the next point where the stream is idle.
 static void drop_graphs(lbmx_batch *b)
 {
-    for (auto &kv : b->graphs) hipGraphExecDestroy(kv.second);
+    for (auto &kv : b->graphs) b->retired.push_back(kv.second);
     b->graphs.clear();
+    if (b->retired.size() > 32 && hipStreamSynchronize(b->stream) == hipSuccess) reap_graphs(b);
 }
 
 // Starts waiting lattices at the current epoch and rewrites the tile table.
@@ -417,6 +438,10 @@ static int commit(lbmx_batch *b)
     if (int rc = upload(b, b->tiles_dev, T.data(), T.size() * sizeof(Tile))) return rc;
     BatchCtl ctl{b->epoch, n_interior, (int)T.size() - n_interior};
     if (int rc = upload(b, b->ctl_dev, &ctl, sizeof ctl)) return rc;
+    const int g5 = grid5_of(n_interior);
+    if (b->tun.order == 5 && g5 != b->grid5) drop_graphs(b);
+    b->n_interior = n_interior;
+    b->grid5 = g5;
     b->dirty = false;
     return LBMX_OK;
 }
@@ -543,7 +568,9 @@ LBMX_API void lbmx_batch_destroy(lbmx_batch *b)
 {
     if (!b) return;
     if (b->stream) hipStreamSynchronize(b->stream);
-    drop_graphs(b);
+    for (auto &kv : b->graphs) b->retired.push_back(kv.second);
+    b->graphs.clear();
+    reap_graphs(b);
     for (auto &s : b->ring) {
         if (s.p) hipHostFree(s.p);
         if (s.ev) hipEventDestroy(s.ev);
@@ -649,6 +676,7 @@ LBMX_API int lbmx_batch_synchronize(lbmx_batch *b)
 {
     if (!b) return lbmx_fail(LBMX_ERR_ARG, "null argument");
     HIPCHK(hipStreamSynchronize(b->stream));
+    reap_graphs(b);
     return LBMX_OK;
 }
 
@@ -703,7 +731,7 @@ LBMX_API int lbmx_batch_set_tuning(lbmx_batch *b, const lbmx_tuning *t)
         t->groups_per_cu > 32)
         return lbmx_fail(LBMX_ERR_ARG, "invalid tuning (threads %d, tile %d, groups/CU %d)",
                          t->threads, t->tile, t->groups_per_cu);
-    if (t->order < 0 || t->order > 4) return lbmx_fail(LBMX_ERR_ARG, "tile order must be 0 .. 4");
+    if (t->order < 0 || t->order > 5) return lbmx_fail(LBMX_ERR_ARG, "tile order must be 0 .. 5");
     b->tun = *t;
     drop_graphs(b);
     b->dirty = true;
@@ -737,15 +765,15 @@ LBMX_API int lbmx_batch_tune(lbmx_batch *b, int trial_steps, lbmx_tuning *chosen
     const long long epoch0 = b->epoch;
     const bool graphs0 = b->use_graphs;
     b->use_graphs = false;
+    // one tile per workgroup (order 5) at every workgroup / tile shape, and the best
+    // persistent shape as a reference
     std::vector<lbmx_tuning> cand;
-    for (int threads : {128, 256})
-        for (int ilp : {1, 2, 4, 8})
-            for (int g : {2, 4, 8, 16})
-                for (int order : {0, 1, 2, 3, 4}) {
-                    const int tile = threads * ilp;
-                    if (tile < 256) continue;
-                    cand.push_back(lbmx_tuning{threads, tile, g, order});
-                }
+    for (int threads : {64, 128, 256})
+        for (int ilp : {1, 2, 4, 8}) {
+            const int tile = threads * ilp;
+            if (tile >= 256 && tile <= 2048) cand.push_back(lbmx_tuning{threads, tile, 16, 5});
+        }
+    cand.push_back(lbmx_tuning{256, 1024, 16, 0});
     double best = 1e300;
     lbmx_tuning pick = keep;
     int rc = LBMX_OK;

@@ -56,16 +56,21 @@ struct lbmx_batch {
     lbmx::BatchLat *lats_dev = nullptr;
     lbmx::Tile *tiles_dev = nullptr;
     long long tiles_cap = 0;
+    int n_interior = 0;   // interior tiles of the committed table
+    int grid5 = 0;        // interior grid of tile order 5 (n_interior rounded up)
     lbmx::BatchCtl *ctl_dev = nullptr;
     std::vector<BatchSlot> slots;
     std::vector<int> order;          // lattices in tile-table order
     std::vector<lbmx::Tile> tile_host;
     long long epoch = 0;             // host mirror of ctl->epoch (steps enqueued)
     bool dirty = true;
-    lbmx_tuning tun{256, 512, 8, 0};
+    lbmx_tuning tun{256, 512, 16, 5};
     int chunk = 32;
     bool use_graphs = true;
     std::map<int, hipGraphExec_t> graphs;
+    // graphs replaced while launches of them may still be queued on the stream:
+    // destroyed once the stream is known to be idle
+    std::vector<hipGraphExec_t> retired;
     std::vector<BatchStaging> ring;
     int ring_next = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -140,7 +140,8 @@ def equal_runs(r1, r2):
 
 @pytest.mark.parametrize("tuning,chunk", [((64, 256, 2, 0), 1), ((128, 1024, 16, 1), 7),
                                           ((256, 2048, 4, 2), 64), ((256, 512, 8, 3), 5),
-                                          ((128, 256, 16, 4), 32)])
+                                          ((128, 256, 16, 4), 32), ((64, 256, 16, 5), 3),
+                                          ((256, 1024, 16, 5), 9)])
 def test_results_independent_of_launch_shape(tuning, chunk):
     cs = mixed_cases()
     ref = run_batch(cs, native.F32_SHIFTED)
