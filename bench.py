@@ -160,8 +160,8 @@ def cylinder_lattice(precision):
 def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True, precision=None):
     """The batched workload: n independent lattices (Poiseuille / Couette / cavity /
     Taylor-Green, sizes uniform in {64..1024} x {64..512}) advanced `steps` steps in
-    FP32-shifted storage with the collision in FP64 (LBMX_F32_WIDE), one persistent
-    HIP-graph-captured launch pair per step.
+    FP32-shifted storage with the collision in FP64 (LBMX_F32_WIDE), one
+    HIP-graph-captured launch pair per step, autotuned launch shape.
     Baseline: the same lattices one after another on the single-lattice FP64 path
     (its plain per-step launches), timed on a seeded random sample of `sample`
     lattices and scaled by n/sample (an unbiased estimate of the full sequential
