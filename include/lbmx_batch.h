@@ -5,8 +5,8 @@
  * The reference runs one lattice per process (cases/<case>/<case>.c main loop,
  * one main_lbm call per step, src/core/main_lbm.c:5-62).  This header adds what
  * that design cannot express and has no counterpart in the reference: a batch
- * of lattices of uneven sizes whose time steps run as ONE persistent launch per
- * step (captured in a HIP graph), and a server that admits solve requests into
+ * of lattices of uneven sizes whose time steps run as one launch pair per step
+ * over all of them (captured in a HIP graph), and a server that admits solve requests into
  * such a batch while it runs (continuous batching).
  *
  * Each lattice is described by an lbmx_case: a program made of the reference's

@@ -1024,7 +1024,36 @@ static int launch_step(lbmx_lattice *L)
     const bool fused = L->nlinks > 0 && !s.exact;
     const int link_grid = (L->npartials + L->nxcd - 1) / L->nxcd * L->nxcd;
     const int gridA = grid + (fused ? link_grid : 0);
-    switch (L->geom) {
+    static int scatter = -1;   // LBMX_PUSH_SCATTER=1: the shifted-store k_bulk in fast mode too
+    if (scatter < 0) scatter = getenv("LBMX_PUSH_SCATTER") ? 1 : 0;
+    if (!s.exact && !scatter) {
+        // destination-aligned push, one row tile per workgroup (k_bulk_al)
+        const long long tiles = ba.ntiles;
+        const int nl = fused ? L->npartials : 0;
+        if (tiles + nl > 0x7fffffffLL) return lbmx_fail(LBMX_ERR_ARG, "lattice too large");
+        const dim3 gal((unsigned)(tiles + nl));
+        switch (L->geom) {
+        case 0:
+            hipLaunchKernelGGL((k_bulk_al<T, 0>), gal, dim3(kAlignThreads), 0, L->stream, S, D, ba);
+            break;
+        case 1:
+            if (fused)
+                hipLaunchKernelGGL((k_bulk_links_al<T, 1>), gal, dim3(kAlignThreads), 0, L->stream,
+                                   S, D, ba, la, nl);
+            else
+                hipLaunchKernelGGL((k_bulk_al<T, 1>), gal, dim3(kAlignThreads), 0, L->stream, S, D,
+                                   ba);
+            break;
+        default:
+            if (fused)
+                hipLaunchKernelGGL((k_bulk_links_al<T, 2>), gal, dim3(kAlignThreads), 0, L->stream,
+                                   S, D, ba, la, nl);
+            else
+                hipLaunchKernelGGL((k_bulk_al<T, 2>), gal, dim3(kAlignThreads), 0, L->stream, S, D,
+                                   ba);
+            break;
+        }
+    } else switch (L->geom) {
     case 0:
         hipLaunchKernelGGL((k_bulk<T, 0>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S, D, ba);
         break;

@@ -574,6 +574,193 @@ __global__ void __launch_bounds__(kBulkThreads) k_bulk_links(T *__restrict__ S,
         bulk_tiles<T, GEOM>(S, D, ab, b - link_grid, (int)gridDim.x - link_grid);
 }
 
+// ---- the same push with destination-aligned stores -------------------------------
+// k_bulk's stores D[k][x + c_k] are shifted by one element for c_k,x != 0: every
+// store instruction covers partial cache lines at both ends, which costs most in
+// FP32 (4-byte lanes).  Here a workgroup owns the 256 sources x0 .. x0+255 of one
+// row y and stores, for each direction k, the destinations x0 .. x0+255 of row
+// y + c_k,y -- whole aligned lines -- taking the value for destination x from
+// source x - c_k,x: its own lane, a neighbouring lane through LDS, or one of two
+// halo sources (x0 - 1, last + 1) whose collision lanes 0 / last compute once more.
+// Every slot (destination, k) is written by exactly the one lane that owns it, and
+// exactly when k_bulk writes it (source in the domain and fluid, destination
+// fluid), with the same value, so the two kernels are interchangeable; the
+// half-way bounce-back into a fluid node's own slot (no interpolation) is still
+// written by the node's own lane.  Not for LBMX_EXACT_STATE (S stays untouched).
+constexpr int kAlignThreads = 256;
+
+// Post-collision populations of fluid node n, as bulk_tiles computes them.
+template <typename T>
+__device__ __forceinline__ void collide_at(const T *__restrict__ S, const BulkArgs &a, size_t n,
+                                           double om, T p[Q])
+{
+    const Geo &g = a.g;
+#pragma unroll
+    for (int k = 0; k < Q; k++) p[k] = S[k * g.plane + n];
+    if constexpr (sizeof(T) == 8) {
+        double *f = reinterpret_cast<double *>(p);
+        const Moments m = a.s.lag_arrays ? Moments{a.F.R[n], a.F.U[n], a.F.V[n]}
+                                         : moments_ref(f, a.s.Fx_lag, a.s.Fy_lag);
+        collide_ref(f, m, om, a.s.force, a.s.Fx, a.s.Fy);
+    } else if (g.wide) {
+        float *q = reinterpret_cast<float *>(p);
+        const MomentsW m = a.s.lag_arrays ? MomentsW{a.F.R[n] - g.rho0, a.F.U[n], a.F.V[n]}
+                                          : moments_wide(q, g.rho0, a.s.Fx_lag, a.s.Fy_lag);
+        collide_wide(q, m, g.rho0, om, a.s.force, a.s.Fx, a.s.Fy);
+    } else {
+        float *q = reinterpret_cast<float *>(p);
+        const float r0 = (float)g.rho0;
+        MomentsF m;
+        if (a.s.lag_arrays) {
+            m.dr = (float)(a.F.R[n] - g.rho0);
+            m.ux = (float)a.F.U[n];
+            m.uy = (float)a.F.V[n];
+        } else {
+            m = moments_f32(q, r0, (float)a.s.Fx_lag, (float)a.s.Fy_lag);
+        }
+        collide_f32(q, m, r0, (float)om, a.s.force, (float)a.s.Fx, (float)a.s.Fy);
+    }
+}
+
+template <int GEOM>
+__device__ __forceinline__ double omega_of(const Geo &g, size_t n)
+{
+    if constexpr (GEOM == 0) return g.om;
+    else if constexpr (GEOM == 1) return g.lut[g.cls[n]];
+    else return g.om_arr[n];
+}
+
+// LDS slot of direction k with c_k,x = +1 (k = 1, 5, 8 -> 0, 1, 2) or -1
+// (k = 3, 6, 7 -> 3, 4, 5)
+__device__ __forceinline__ constexpr int xdir_slot(int k)
+{
+    return k == 1 ? 0 : k == 5 ? 1 : k == 8 ? 2 : k == 3 ? 3 : k == 6 ? 4 : 5;
+}
+
+template <typename T, int GEOM>
+__device__ __forceinline__ void aligned_tile(const T *__restrict__ S, T *__restrict__ D,
+                                             const BulkArgs &a, int tile)
+{
+    static_assert(kAlignThreads == kBulkThreads, "row tiles as BulkArgs::tiles_per_row counts them");
+    __shared__ T lx[6][kAlignThreads + 2];
+    __shared__ int lok[kAlignThreads + 2];
+    const Geo &g = a.g;
+    const size_t P = g.plane;
+    const int y = tile / a.tiles_per_row;
+    const int x0 = (tile % a.tiles_per_row) * kAlignThreads;
+    const int t = threadIdx.x;
+    const int x = x0 + t;
+    const int tl = min(x0 + kAlignThreads, g.Nx) - 1 - x0;   // last valid lane
+    const bool valid = x < g.Nx;
+    const size_t n = (size_t)y * g.pitch + x;
+
+    T p[Q];
+    bool ok = false;
+    if (valid) {
+        ok = (GEOM == 0) || g.cls[n] != 0;
+        if (ok) collide_at(S, a, n, omega_of<GEOM>(g, n), p);
+        lok[t + 1] = ok;
+        if (ok) {
+            lx[0][t + 1] = p[1];
+            lx[1][t + 1] = p[5];
+            lx[2][t + 1] = p[8];
+            lx[3][t + 1] = p[3];
+            lx[4][t + 1] = p[6];
+            lx[5][t + 1] = p[7];
+        }
+    }
+    // halo sources: x0 - 1 (read by lane 0 for c_k,x = +1) into slot 0, and
+    // x0 + tl + 1 (read by lane tl for c_k,x = -1) into slot tl + 2
+    if (t == 0 || t == tl) {
+        for (int side = 0; side < 2; side++) {
+            if ((side == 0 && t != 0) || (side == 1 && t != tl)) continue;
+            int xs = side == 0 ? x0 - 1 : x0 + tl + 1;
+            bool hok = true;
+            if (xs < 0 || xs >= g.Nx) {
+                if (g.per_x) xs += xs < 0 ? g.Nx : -g.Nx;
+                else hok = false;
+            }
+            const size_t ns = (size_t)y * g.pitch + xs;
+            if (hok && GEOM != 0 && g.cls[ns] == 0) hok = false;
+            const int slot = side == 0 ? 0 : tl + 2;
+            lok[slot] = hok;
+            if (hok) {
+                T h[Q];
+                collide_at(S, a, ns, omega_of<GEOM>(g, ns), h);
+                lx[0][slot] = h[1];
+                lx[1][slot] = h[5];
+                lx[2][slot] = h[8];
+                lx[3][slot] = h[3];
+                lx[4][slot] = h[6];
+                lx[5][slot] = h[7];
+            }
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+
+    // half-way bounce-back into the own slots (no interpolation), as k_bulk
+    if (GEOM != 0 && ok && !a.ibb) {
+#pragma unroll
+        for (int k = 1; k < Q; k++) {
+            int xd = x + D2Q9::cx[k], yd = y + D2Q9::cy[k];
+            if (xd < 0 || xd >= g.Nx) {
+                if (!g.per_x) continue;
+                xd += xd < 0 ? g.Nx : -g.Nx;
+            }
+            if (yd < 0 || yd >= g.Ny) {
+                if (!g.per_y) continue;
+                yd += yd < 0 ? g.Ny : -g.Ny;
+            }
+            if (g.cls[(size_t)yd * g.pitch + xd] == 0) D[D2Q9::opp[k] * P + n] = p[k];
+        }
+    }
+
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
+        int yd = y + cy;
+        if (yd < 0 || yd >= g.Ny) {
+            if (!g.per_y) continue;          // the whole row of destinations is outside
+            yd += yd < 0 ? g.Ny : -g.Ny;
+        }
+        const size_t nd = (size_t)yd * g.pitch + x;
+        bool w;
+        T v;
+        if (cx == 0) {
+            w = ok;
+            v = p[k];
+        } else {
+            const int slot = cx > 0 ? t : t + 2;   // source x - c_k,x
+            w = lok[slot];
+            v = lx[xdir_slot(k)][slot];
+        }
+        if (GEOM != 0 && k != 0 && w && g.cls[nd] == 0) w = false;   // solid destination
+        if (w) D[k * P + nd] = v;
+    }
+}
+
+template <typename T, int GEOM>
+__global__ void __launch_bounds__(kAlignThreads) k_bulk_al(T *__restrict__ S, T *__restrict__ D,
+                                                          BulkArgs a)
+{
+    aligned_tile<T, GEOM>(S, D, a, blockIdx.x);
+}
+
+// k_bulk_al and the links in one launch, link blocks first (as k_bulk_links).
+template <typename T, int GEOM>
+__global__ void __launch_bounds__(kAlignThreads) k_bulk_links_al(T *__restrict__ S,
+                                                                T *__restrict__ D, BulkArgs ab,
+                                                                LinkArgs al, int nlink)
+{
+    static_assert(kAlignThreads == kLinkThreads, "one block size for both parts");
+    const int b = blockIdx.x;
+    if (b < nlink)
+        links_block(S, D, al, b);
+    else
+        aligned_tile<T, GEOM>(S, D, ab, b - nlink);
+}
+
 // ---- k_bc: the boundary table, in order, in one workgroup ----------------------
 struct BcDesc {
     int type, index;
