@@ -505,6 +505,7 @@ extern "C" int lbmx_set_boundaries(lbmx_lattice *L, const int *types, const int 
         edge.push_back((lbmx_bc_on_column(t) || t == LBMX_BC_PERIODIC_X) ? L->Ny : L->Nx);
     }
     L->prog_edge = edge;
+    L->prog_desc = prog;
     hipFree(L->bc_dev);
     L->bc_dev = nullptr;
     L->nprog = (int)prog.size();
@@ -980,6 +981,32 @@ extern "C" int lbmx_tail_plan(int Nx, int Ny, int per_x, int per_y, const int *m
     return (int)plan.size();
 }
 
+template <typename T, bool W>
+static void launch_aligned(lbmx_lattice *L, T *S, T *D, const BulkArgs &ba, const LinkArgs &la,
+                           int nl, bool fused, unsigned blocks)
+{
+    const dim3 grid(blocks), block(kAlignThreads);
+    switch (L->geom) {
+    case 0:
+        hipLaunchKernelGGL((k_bulk_al<T, 0, W>), grid, block, 0, L->stream, S, D, ba);
+        break;
+    case 1:
+        if (fused)
+            hipLaunchKernelGGL((k_bulk_links_al<T, 1, W>), grid, block, 0, L->stream, S, D, ba,
+                               la, nl);
+        else
+            hipLaunchKernelGGL((k_bulk_al<T, 1, W>), grid, block, 0, L->stream, S, D, ba);
+        break;
+    default:
+        if (fused)
+            hipLaunchKernelGGL((k_bulk_links_al<T, 2, W>), grid, block, 0, L->stream, S, D, ba,
+                               la, nl);
+        else
+            hipLaunchKernelGGL((k_bulk_al<T, 2, W>), grid, block, 0, L->stream, S, D, ba);
+        break;
+    }
+}
+
 template <typename T>
 static int launch_step(lbmx_lattice *L)
 {
@@ -1031,28 +1058,10 @@ static int launch_step(lbmx_lattice *L)
         const long long tiles = ba.ntiles;
         const int nl = fused ? L->npartials : 0;
         if (tiles + nl > 0x7fffffffLL) return lbmx_fail(LBMX_ERR_ARG, "lattice too large");
-        const dim3 gal((unsigned)(tiles + nl));
-        switch (L->geom) {
-        case 0:
-            hipLaunchKernelGGL((k_bulk_al<T, 0>), gal, dim3(kAlignThreads), 0, L->stream, S, D, ba);
-            break;
-        case 1:
-            if (fused)
-                hipLaunchKernelGGL((k_bulk_links_al<T, 1>), gal, dim3(kAlignThreads), 0, L->stream,
-                                   S, D, ba, la, nl);
-            else
-                hipLaunchKernelGGL((k_bulk_al<T, 1>), gal, dim3(kAlignThreads), 0, L->stream, S, D,
-                                   ba);
-            break;
-        default:
-            if (fused)
-                hipLaunchKernelGGL((k_bulk_links_al<T, 2>), gal, dim3(kAlignThreads), 0, L->stream,
-                                   S, D, ba, la, nl);
-            else
-                hipLaunchKernelGGL((k_bulk_al<T, 2>), gal, dim3(kAlignThreads), 0, L->stream, S, D,
-                                   ba);
-            break;
-        }
+        if (sizeof(T) == 4 && g.wide)
+            launch_aligned<T, true>(L, S, D, ba, la, nl, fused, (unsigned)(tiles + nl));
+        else
+            launch_aligned<T, false>(L, S, D, ba, la, nl, fused, (unsigned)(tiles + nl));
     } else switch (L->geom) {
     case 0:
         hipLaunchKernelGGL((k_bulk<T, 0>), dim3(grid), dim3(kBulkThreads), 0, L->stream, S, D, ba);
@@ -1083,7 +1092,7 @@ static int launch_step(lbmx_lattice *L)
 
     // boundary table, force outputs and stale edges: k_tail launches as planned
     if (!L->tail_planned) lbmx_plan_tail(L);
-    TailArgs ta;
+    TailArgs ta{};
     ta.bc.g = g;
     ta.bc.s = s;
     ta.bc.F = F;
@@ -1106,6 +1115,7 @@ static int launch_step(lbmx_lattice *L)
         ta.block0[0] = 0;
         for (int b : tl.entry) {
             ta.entry[ta.njobs] = b;
+            ta.desc[ta.njobs] = L->prog_desc[b];
             ta.block0[ta.njobs + 1] =
                 ta.block0[ta.njobs] + (L->prog_edge[b] + kTailThreads - 1) / kTailThreads;
             ta.njobs++;

@@ -64,6 +64,7 @@ struct lbmx_lattice {
     int nprog = 0;
     std::vector<int> prog_edge;   // edge length of each program entry
     std::vector<int> prog_type, prog_index;
+    std::vector<lbmx::BcDesc> prog_desc;   // host copy of bc_dev
     // host copy of the solid mask, Ny x Nx row-major (empty: no solid node)
     std::vector<uint8_t> hsolid;
     // the work after streaming as k_tail launches: program entries per launch and

@@ -589,20 +589,19 @@ __global__ void __launch_bounds__(kBulkThreads) k_bulk_links(T *__restrict__ S,
 // written by the node's own lane.  Not for LBMX_EXACT_STATE (S stays untouched).
 constexpr int kAlignThreads = 256;
 
-// Post-collision populations of fluid node n, as bulk_tiles computes them.
-template <typename T>
-__device__ __forceinline__ void collide_at(const T *__restrict__ S, const BulkArgs &a, size_t n,
-                                           double om, T p[Q])
+// Post-collision populations of fluid node n, as bulk_tiles computes them; W
+// selects the FP32-storage collision at compile time (true: FP64 arithmetic,
+// g.wide), so each kernel instance carries one collision.
+template <typename T, bool W>
+__device__ __forceinline__ void collide_loaded(const BulkArgs &a, size_t n, double om, T p[Q])
 {
     const Geo &g = a.g;
-#pragma unroll
-    for (int k = 0; k < Q; k++) p[k] = S[k * g.plane + n];
     if constexpr (sizeof(T) == 8) {
         double *f = reinterpret_cast<double *>(p);
         const Moments m = a.s.lag_arrays ? Moments{a.F.R[n], a.F.U[n], a.F.V[n]}
                                          : moments_ref(f, a.s.Fx_lag, a.s.Fy_lag);
         collide_ref(f, m, om, a.s.force, a.s.Fx, a.s.Fy);
-    } else if (g.wide) {
+    } else if constexpr (W) {
         float *q = reinterpret_cast<float *>(p);
         const MomentsW m = a.s.lag_arrays ? MomentsW{a.F.R[n] - g.rho0, a.F.U[n], a.F.V[n]}
                                           : moments_wide(q, g.rho0, a.s.Fx_lag, a.s.Fy_lag);
@@ -622,6 +621,15 @@ __device__ __forceinline__ void collide_at(const T *__restrict__ S, const BulkAr
     }
 }
 
+template <typename T, bool W>
+__device__ __forceinline__ void collide_at(const T *__restrict__ S, const BulkArgs &a, size_t n,
+                                           double om, T p[Q])
+{
+#pragma unroll
+    for (int k = 0; k < Q; k++) p[k] = S[k * a.g.plane + n];
+    collide_loaded<T, W>(a, n, om, p);
+}
+
 template <int GEOM>
 __device__ __forceinline__ double omega_of(const Geo &g, size_t n)
 {
@@ -637,7 +645,7 @@ __device__ __forceinline__ constexpr int xdir_slot(int k)
     return k == 1 ? 0 : k == 5 ? 1 : k == 8 ? 2 : k == 3 ? 3 : k == 6 ? 4 : 5;
 }
 
-template <typename T, int GEOM>
+template <typename T, int GEOM, bool W>
 __device__ __forceinline__ void aligned_tile(const T *__restrict__ S, T *__restrict__ D,
                                              const BulkArgs &a, int tile)
 {
@@ -654,45 +662,64 @@ __device__ __forceinline__ void aligned_tile(const T *__restrict__ S, T *__restr
     const bool valid = x < g.Nx;
     const size_t n = (size_t)y * g.pitch + x;
 
+    // halo sources: x0 - 1 (read by lane 0 for c_k,x = +1) into slot 0, and
+    // x0 + tl + 1 (read by lane tl for c_k,x = -1) into slot tl + 2.  Lanes 0 and
+    // tl load theirs together with their own node, so the two loads overlap; a
+    // one-lane tile (tl == 0) takes both halos in lane 0 afterwards.
+    const auto halo_source = [&](int side, size_t &ns) -> bool {
+        int xs = side == 0 ? x0 - 1 : x0 + tl + 1;
+        if (xs < 0 || xs >= g.Nx) {
+            if (!g.per_x) return false;
+            xs += xs < 0 ? g.Nx : -g.Nx;
+        }
+        ns = (size_t)y * g.pitch + xs;
+        return GEOM == 0 || g.cls[ns] != 0;
+    };
+    const auto put = [&](int slot, const T v[Q]) {
+        lx[0][slot] = v[1];
+        lx[1][slot] = v[5];
+        lx[2][slot] = v[8];
+        lx[3][slot] = v[3];
+        lx[4][slot] = v[6];
+        lx[5][slot] = v[7];
+    };
+    const bool one_lane = (tl == 0);
+    const bool has_h = !one_lane && (t == 0 || t == tl);
+    const int hslot = t == 0 ? 0 : tl + 2;
+    size_t hn = 0;
+    bool hok = false;
+    T h[Q];
+    if (has_h) {
+        hok = halo_source(t == 0 ? 0 : 1, hn);
+        if (hok)
+#pragma unroll
+            for (int k = 0; k < Q; k++) h[k] = S[k * P + hn];
+    }
     T p[Q];
     bool ok = false;
     if (valid) {
         ok = (GEOM == 0) || g.cls[n] != 0;
-        if (ok) collide_at(S, a, n, omega_of<GEOM>(g, n), p);
+        if (ok) collide_at<T, W>(S, a, n, omega_of<GEOM>(g, n), p);
         lok[t + 1] = ok;
-        if (ok) {
-            lx[0][t + 1] = p[1];
-            lx[1][t + 1] = p[5];
-            lx[2][t + 1] = p[8];
-            lx[3][t + 1] = p[3];
-            lx[4][t + 1] = p[6];
-            lx[5][t + 1] = p[7];
+        if (ok) put(t + 1, p);
+    }
+    if (has_h) {
+        lok[hslot] = hok;
+        if (hok) {
+            collide_loaded<T, W>(a, hn, omega_of<GEOM>(g, hn), h);
+            put(hslot, h);
         }
     }
-    // halo sources: x0 - 1 (read by lane 0 for c_k,x = +1) into slot 0, and
-    // x0 + tl + 1 (read by lane tl for c_k,x = -1) into slot tl + 2
-    if (t == 0 || t == tl) {
+    if (one_lane && t == 0) {
         for (int side = 0; side < 2; side++) {
-            if ((side == 0 && t != 0) || (side == 1 && t != tl)) continue;
-            int xs = side == 0 ? x0 - 1 : x0 + tl + 1;
-            bool hok = true;
-            if (xs < 0 || xs >= g.Nx) {
-                if (g.per_x) xs += xs < 0 ? g.Nx : -g.Nx;
-                else hok = false;
-            }
-            const size_t ns = (size_t)y * g.pitch + xs;
-            if (hok && GEOM != 0 && g.cls[ns] == 0) hok = false;
-            const int slot = side == 0 ? 0 : tl + 2;
-            lok[slot] = hok;
-            if (hok) {
-                T h[Q];
-                collide_at(S, a, ns, omega_of<GEOM>(g, ns), h);
-                lx[0][slot] = h[1];
-                lx[1][slot] = h[5];
-                lx[2][slot] = h[8];
-                lx[3][slot] = h[3];
-                lx[4][slot] = h[6];
-                lx[5][slot] = h[7];
+            size_t ns = 0;
+            const bool sok = halo_source(side, ns);
+            const int slot = side == 0 ? 0 : 2;
+            lok[slot] = sok;
+            if (sok) {
+                T v[Q];
+                collide_at<T, W>(S, a, ns, omega_of<GEOM>(g, ns), v);
+                put(slot, v);
             }
         }
     }
@@ -740,15 +767,15 @@ __device__ __forceinline__ void aligned_tile(const T *__restrict__ S, T *__restr
     }
 }
 
-template <typename T, int GEOM>
+template <typename T, int GEOM, bool W>
 __global__ void __launch_bounds__(kAlignThreads) k_bulk_al(T *__restrict__ S, T *__restrict__ D,
                                                           BulkArgs a)
 {
-    aligned_tile<T, GEOM>(S, D, a, blockIdx.x);
+    aligned_tile<T, GEOM, W>(S, D, a, blockIdx.x);
 }
 
 // k_bulk_al and the links in one launch, link blocks first (as k_bulk_links).
-template <typename T, int GEOM>
+template <typename T, int GEOM, bool W>
 __global__ void __launch_bounds__(kAlignThreads) k_bulk_links_al(T *__restrict__ S,
                                                                 T *__restrict__ D, BulkArgs ab,
                                                                 LinkArgs al, int nlink)
@@ -758,7 +785,7 @@ __global__ void __launch_bounds__(kAlignThreads) k_bulk_links_al(T *__restrict__
     if (b < nlink)
         links_block(S, D, al, b);
     else
-        aligned_tile<T, GEOM>(S, D, ab, b - nlink);
+        aligned_tile<T, GEOM, W>(S, D, ab, b - nlink);
 }
 
 // ---- k_bc: the boundary table, in order, in one workgroup ----------------------
@@ -1135,6 +1162,7 @@ struct TailArgs {
     int njobs;
     int with_force;
     int entry[kTailJobs];           // boundary table entry, or -1 for the edge job
+    BcDesc desc[kTailJobs];         // that entry's descriptor (no device-table load)
     int block0[kTailJobs + 1];
 };
 
@@ -1149,7 +1177,7 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(T *__restrict__ S, T *__r
         const Span sp{(blk - a.block0[j]) * kTailThreads + (int)threadIdx.x,
                       (a.block0[j + 1] - a.block0[j]) * kTailThreads};
         if (a.entry[j] >= 0) {
-            bc_apply((const T *)S, D, a.bc, a.bc.bcs[a.entry[j]], sp);
+            bc_apply((const T *)S, D, a.bc, a.desc[j], sp);
         } else {
             const EdgeArgs ea{a.bc.g, a.bc.s, a.bc.F, a.nrow, a.ncol};
             for (int e = sp.first; e < a.nrow + a.ncol; e += sp.stride) edge_node(S, ea, e);
