@@ -1,0 +1,146 @@
+"""Parity at the benchmark sizes.
+
+The small-lattice parity tests pin every code path bit for bit against the oracle;
+these run the benchmark lattices themselves, with checks that do not need the
+oracle to run the whole workload:
+
+* Taylor-Green 16384^2, FP64: the gather form (multi-step calls, k_pull) and the
+  push form (single-step calls: aligned bulk, boundary launches) are independent
+  kernels; after 12 steps their moments must agree bit for bit, and total mass
+  and momentum must be conserved.  FP32 storage stays within 1e-5 (FP32
+  collision) and 1e-7 (FP64 collision) relative L2 of FP64.
+* The cylinder channel 2048 x 512 (IBB, regularized inlet/outlet, walls): 40
+  steps against the oracle at full size, bit for bit, and the force record.
+* The liner duct 8192 x 256 in the batched engine: 40 FP64 steps against the
+  oracle's liner driver at full size, bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lbc_amd import batch as B
+from lbc_amd import cases, native
+
+pytestmark = pytest.mark.gpu
+
+
+def tgv(N, precision):
+    lat = native.Lattice(N, N, precision=precision, per_x=1, per_y=1)
+    lat.set_uniform(1.0 / 0.8)
+    lat.set_boundaries([("periodic_x", 0), ("periodic_y", 0)])
+    lat.init_taylor_green(0.01, 2 * math.pi / N, 2 * math.pi / N)
+    return lat
+
+
+def test_tgv16384_gather_equals_push_and_conserves():
+    N, steps = 16384, 12
+    lat = tgv(N, native.F64)
+    r0, _, _ = lat.moments()
+    mass0 = float(np.sum(r0))
+    del r0
+    lat.step(steps)                      # gather form
+    ra, ua, va = lat.moments()
+    lat.close()
+    lat = tgv(N, native.F64)
+    for _ in range(steps):
+        lat.step(1)                      # push form, one call per step
+    rb, ub, vb = lat.moments()
+    lat.close()
+    assert np.array_equal(ra, rb) and np.array_equal(ua, ub) and np.array_equal(va, vb)
+    mass = float(np.sum(ra))
+    assert abs(mass - mass0) <= 1e-12 * mass0, (mass, mass0)
+    # no body force: total momentum stays zero (the vortex is antisymmetric)
+    px = float(np.sum(ra * ua))
+    py = float(np.sum(ra * va))
+    scale = 0.01 * N * N
+    assert abs(px) < 1e-9 * scale and abs(py) < 1e-9 * scale, (px, py)
+
+
+def test_tgv16384_fp32_close_to_fp64():
+    N, steps = 16384, 12
+    lat = tgv(N, native.F64)
+    lat.step(steps)
+    _, u64, v64 = lat.moments()
+    lat.close()
+    # relative L2 velocity error bars: 1e-5 with the FP32 collision, 1e-7 with the FP64
+    # collision on FP32-stored deviations
+    bar = {native.F32_SHIFTED: 1e-5, native.F32_WIDE: 1e-7}
+    for prec in (native.F32_SHIFTED, native.F32_WIDE):
+        lat = tgv(N, prec)
+        lat.step(steps)
+        _, u32, v32 = lat.moments()
+        lat.close()
+        err = math.sqrt(float(((u32 - u64) ** 2 + (v32 - v64) ** 2).sum())
+                        / float((u64 ** 2 + v64 ** 2).sum()))
+        print(f"16384^2, {steps} steps, precision {prec}: relative L2 velocity error {err:.2e}")
+        assert err < bar[prec]
+
+
+def test_cylinder_full_size_matches_oracle():
+    Nx, Ny, R, xc, yc, umax = 2048, 512, 32.0, 512.0, 256.0, 0.05
+    steps = 40
+    lat, par = cases.cylinder_channel(Nx, Ny, R, xc, yc, umax, 100.0, native.F64)
+    lat.set_force_record(steps)
+    lat.step(steps)
+    rho, u, v = lat.moments()
+    rec = lat.force_record()
+    lat.close()
+
+    j, i = np.mgrid[0:Ny, 0:Nx]
+    walls = (j == 0) | (j == Ny - 1)
+    d = np.sqrt((i - xc) ** 2 + (j - yc) ** 2) - R
+    o = O.Lattice(Nx, Ny)
+    o.mask = (walls | (d <= 0)).astype(np.int32)
+    o.phi = np.where(walls, -1e30, d)
+    o.omega = np.full((Ny, Nx), 1.0 / par["tau"])
+    o.f = cases.equilibrium(np.ones((Ny, Nx)), np.zeros((Ny, Nx)), np.zeros((Ny, Nx)))
+    o.f_new = o.f.copy()
+    y = np.arange(Ny) - 0.5
+    H = Ny - 2.0
+    uin = np.clip(4 * umax * y * (H - y) / H ** 2, 0.0, None)
+    bcs = [("velocity_inlet_regularized", 0, uin, np.zeros(Ny)),
+           ("pressure_outlet_regularized", Nx - 1, np.ones(Ny), None)]
+    L, D = O.oracle_run(o, steps, bcs, 0, 0, 0.0, 0.0, 1)
+    same = lambda a, b: np.array_equal(a, b, equal_nan=True)
+    assert same(rho, o.rho) and same(u, o.u) and same(v, o.v)
+    # forces: the same ~13,000 link terms summed in another (fixed) order; the lift
+    # of this symmetric channel is zero up to the cancellation of the two walls'
+    # sums (each ~1e3), hence the absolute bar
+    np.testing.assert_allclose(rec[:, 1], L, rtol=1e-11, atol=1e-10)
+    np.testing.assert_allclose(rec[:, 2], D, rtol=1e-11, atol=1e-10)
+
+
+def test_liner_full_size_batch_matches_oracle():
+    import ctypes as C
+    steps = 40
+    imp = cases.LinerSpec(R=1.0, f_res=1.0 / 40.0, M=8.0).impedance()
+    c = B.liner_duct(8192, 256, steps, imp, tau=0.51, amp=1e-3, x0=1024.0, halfwidth=20.0,
+                     liner=(3072, 5120), sponge=192, probes=[(3584, 0), (6144, 128)])
+    b = B.Batch(native.F64, budget=B.case_bytes(c, native.F64) + (64 << 20))
+    i = b.add(c)
+    b.run(steps)
+    b.synchronize()
+    rho, _, _ = b.moments(i)
+    hist = b.probes(i)
+    b.close()
+
+    cfg = O.OrcLinerCfg()
+    cfg.Nx, cfg.Ny = c.Nx, c.Ny
+    cfg.liner_x0, cfg.liner_x1 = c.gap
+    cfg.sponge = c.sponge
+    cfg.probe_x, cfg.probe_y = c.probes[1]
+    cfg.tau, cfg.tau_sponge, cfg.rho0 = c.tau, c.tau_sponge, c.rho0
+    cfg.amp, cfg.x0, cfg.halfwidth, cfg.gain = c.amp, c.x0, c.halfwidth, c.gain
+    cfg.imp.Zinf = c.liner["Zinf"]
+    for n in ("A", "lambdas", "B", "C", "alpha", "beta"):
+        arr = getattr(cfg.imp, n)
+        arr[0], arr[1] = c.liner[n]
+    probe = np.zeros(steps)
+    rho_ref = np.zeros((c.Ny, c.Nx))
+    dp = C.POINTER(C.c_double)
+    O.ORACLE.orc_liner_run(C.byref(cfg), steps, probe.ctypes.data_as(dp), None, None,
+                           rho_ref.ctypes.data_as(dp))
+    assert np.array_equal(rho, rho_ref, equal_nan=True)
+    assert np.array_equal(hist[:steps, 1], probe)
