@@ -1207,6 +1207,52 @@ static int pull_step(lbmx_lattice *L)
     return LBMX_OK;
 }
 
+// Two gather-form steps in one launch (k_pull2); the buffer of f*_{s+1} is not
+// written, so callers end every run of steps with pull_step.
+template <typename T, bool W, int TY>
+static void launch_pull2(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
+{
+    const unsigned blocks = (unsigned)a.tiles_x * (unsigned)((L->Ny + TY - 1) / TY);
+    hipLaunchKernelGGL((k_pull2<T, W, TY>), dim3(blocks), dim3(kPull2Threads), 0, L->stream, S,
+                       D, a);
+}
+
+template <typename T>
+static int pull_step2(lbmx_lattice *L)
+{
+    const Geo g = lbmx_geo(L);
+    StepFlags s = lbmx_flags(L);
+    Pull2Args a;
+    a.g = g;
+    a.force = s.force;
+    a.Fx = s.Fx;
+    a.Fy = s.Fy;
+    a.Fx_lag = s.Fx_lag;
+    a.Fy_lag = s.Fy_lag;
+    a.tiles_x = (L->Nx + kPull2TX - 1) / kPull2TX;
+    const T *S = (const T *)L->pop[L->pstar];
+    T *D = (T *)L->pop[L->pstar ^ 1];
+    // tile heights (LDS of a workgroup: 9 x 66 x (TY + 2) values, under 64 KB) as
+    // measured fastest at 16384^2 among 2-24: FP64 8 (3.72 ms per step; 4: 3.82,
+    // 10: 4.23), FP32 8 (1.91; 16: 1.94, 24: 2.25), FP32 storage + FP64
+    // collision 16 (2.57; 8: 2.72)
+    if constexpr (sizeof(T) == 8)
+        launch_pull2<T, false, 8>(L, S, D, a);
+    else if (g.wide)
+        launch_pull2<T, true, 16>(L, S, D, a);
+    else
+        launch_pull2<T, false, 8>(L, S, D, a);
+    HIPCHK(hipGetLastError());
+    L->pstar ^= 1;
+    s.Fx_lag = L->cfg.Fx;
+    s.Fy_lag = L->cfg.Fy;
+    L->last = s;
+    L->Fx_lag = L->cfg.Fx;
+    L->Fy_lag = L->cfg.Fy;
+    L->steps += 2;
+    return LBMX_OK;
+}
+
 int lbmx_ensure_push(lbmx_lattice *L)
 {
     if (!L->pull) return LBMX_OK;
@@ -1243,7 +1289,17 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
         int rc = lbmx_ensure_push(L);
         if (rc) return rc;
     }
-    for (int i = 0; i < nsteps; i++) {
+    int i = 0;
+    static int temporal = -1;   // LBMX_NO_TEMPORAL=1: one launch per step in gather form too
+    if (temporal < 0) temporal = getenv("LBMX_NO_TEMPORAL") ? 0 : 1;
+    if (L->pull && temporal) {
+        // pairs of steps while at least three remain: the run ends with pull_step
+        for (; nsteps - i >= 3; i += 2) {
+            int rc = f64 ? pull_step2<double>(L) : pull_step2<float>(L);
+            if (rc) return rc;
+        }
+    }
+    for (; i < nsteps; i++) {
         int rc;
         if (L->pull)
             rc = f64 ? pull_step<double>(L) : pull_step<float>(L);

@@ -117,6 +117,86 @@ __global__ void __launch_bounds__(256) k_collide_inplace(T *__restrict__ S, Pull
     for (int k = 0; k < Q; k++) S[k * g.plane + n] = p[k];
 }
 
+// ---- two steps per launch (temporal blocking) ------------------------------------
+// A workgroup owns a TX x TY tile of destinations.  Step 1 gathers and collides the
+// tile plus a one-node ring (periodic wrap) from S = f*_s in global memory into
+// LDS; step 2 gathers the tile from LDS, collides and stores f*_{s+2}.  Each pass
+// over the lattice then reads f*_s once (plus the rings, mostly from L2) and
+// writes f*_{s+2} once: about 60 % of the HBM traffic of two k_pull launches for
+// the same two steps.  The arithmetic is k_pull's, on the same values, so the
+// result is bit-identical to two k_pull launches.  The intermediate f*_{s+1} is
+// never stored, so a run of steps ends with a k_pull step (the buffer pair then
+// holds f*_{k-1} and f*_k, as lbmx_ensure_push needs).
+template <typename T, bool W>
+__device__ __forceinline__ void pull_collide(T p[Q], const Geo &g, double Fx_lag, double Fy_lag,
+                                             bool force, double Fx, double Fy)
+{
+    if constexpr (sizeof(T) == 8) {
+        double *f = reinterpret_cast<double *>(p);
+        const Moments m = moments_ref(f, Fx_lag, Fy_lag);
+        collide_ref(f, m, g.om, force, Fx, Fy);
+    } else if constexpr (W) {
+        float *q = reinterpret_cast<float *>(p);
+        const MomentsW m = moments_wide(q, g.rho0, Fx_lag, Fy_lag);
+        collide_wide(q, m, g.rho0, g.om, force, Fx, Fy);
+    } else {
+        float *q = reinterpret_cast<float *>(p);
+        const float r0 = (float)g.rho0;
+        const MomentsF m = moments_f32(q, r0, (float)Fx_lag, (float)Fy_lag);
+        collide_f32(q, m, r0, (float)g.om, force, (float)Fx, (float)Fy);
+    }
+}
+
+struct Pull2Args {
+    Geo g;
+    int force;
+    double Fx, Fy;            // body force of both steps
+    double Fx_lag, Fy_lag;    // force the first step's lagged moments were formed with
+    int tiles_x;              // tiles per row of tiles
+};
+
+constexpr int kPull2Threads = 256;
+constexpr int kPull2TX = 64;
+
+template <typename T, bool W, int TY>
+__global__ void __launch_bounds__(kPull2Threads) k_pull2(const T *__restrict__ S,
+                                                        T *__restrict__ D, Pull2Args a)
+{
+    constexpr int RX = kPull2TX + 2, RN = RX * (TY + 2);
+    __shared__ T buf[Q][RN];
+    const Geo &g = a.g;
+    const int x0 = (blockIdx.x % a.tiles_x) * kPull2TX;
+    const int y0 = (blockIdx.x / a.tiles_x) * TY;
+
+    // step 1: the tile and its ring, from global memory
+    for (int r = threadIdx.x; r < RN; r += kPull2Threads) {
+        int x = x0 - 1 + r % RX, y = y0 - 1 + r / RX;
+        x = (x % g.Nx + g.Nx) % g.Nx;
+        y = (y % g.Ny + g.Ny) % g.Ny;
+        T p[Q];
+        gather_node(S, g, x, y, p);
+        pull_collide<T, W>(p, g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
+#pragma unroll
+        for (int k = 0; k < Q; k++) buf[k][r] = p[k];
+    }
+    __syncthreads();
+
+    // step 2: the tile, from LDS
+    for (int q = threadIdx.x; q < kPull2TX * TY; q += kPull2Threads) {
+        const int tx = q % kPull2TX, ty = q / kPull2TX;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= g.Nx || y >= g.Ny) continue;
+        T p[Q];
+#pragma unroll
+        for (int k = 0; k < Q; k++)
+            p[k] = buf[k][(ty + 1 - D2Q9::cy[k]) * RX + (tx + 1 - D2Q9::cx[k])];
+        pull_collide<T, W>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
+        const size_t n = (size_t)y * g.pitch + x;
+#pragma unroll
+        for (int k = 0; k < Q; k++) store_nt(&D[k * g.plane + n], p[k]);
+    }
+}
+
 // f_k = gather(f*_{k-1}) (streaming only).
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather(const T *__restrict__ S, T *__restrict__ D, Geo g)
