@@ -1217,6 +1217,50 @@ static void launch_pull2(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
                        D, a);
 }
 
+template <typename T, bool W, int TY>
+static void launch_pull3(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
+{
+    const unsigned blocks = (unsigned)a.tiles_x * (unsigned)((L->Ny + TY - 1) / TY);
+    hipLaunchKernelGGL((k_pull3<T, W, TY>), dim3(blocks), dim3(kPull2Threads), 0, L->stream, S,
+                       D, a);
+}
+
+// Three gather-form steps in one launch (k_pull3); as pull_step2 otherwise.
+template <typename T>
+static int pull_step3(lbmx_lattice *L)
+{
+    const Geo g = lbmx_geo(L);
+    StepFlags s = lbmx_flags(L);
+    Pull2Args a;
+    a.g = g;
+    a.force = s.force;
+    a.Fx = s.Fx;
+    a.Fy = s.Fy;
+    a.Fx_lag = s.Fx_lag;
+    a.Fy_lag = s.Fy_lag;
+    a.tiles_x = (L->Nx + kPull2TX - 1) / kPull2TX;
+    const T *S = (const T *)L->pop[L->pstar];
+    T *D = (T *)L->pop[L->pstar ^ 1];
+    // tile heights measured at 16384^2 (ms per step): FP32 8 (1.70; 4: 1.69, 16:
+    // 1.70), FP32 storage + FP64 collision 16 (2.38; 8: 2.57), FP64 8 (3.81; the
+    // two-step kernel, 3.72, is faster there and is the FP64 default)
+    if constexpr (sizeof(T) == 8)
+        launch_pull3<T, false, 8>(L, S, D, a);
+    else if (g.wide)
+        launch_pull3<T, true, 16>(L, S, D, a);
+    else
+        launch_pull3<T, false, 8>(L, S, D, a);
+    HIPCHK(hipGetLastError());
+    L->pstar ^= 1;
+    s.Fx_lag = L->cfg.Fx;
+    s.Fy_lag = L->cfg.Fy;
+    L->last = s;
+    L->Fx_lag = L->cfg.Fx;
+    L->Fy_lag = L->cfg.Fy;
+    L->steps += 3;
+    return LBMX_OK;
+}
+
 template <typename T>
 static int pull_step2(lbmx_lattice *L)
 {
@@ -1292,8 +1336,20 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
     int i = 0;
     static int temporal = -1;   // LBMX_NO_TEMPORAL=1: one launch per step in gather form too
     if (temporal < 0) temporal = getenv("LBMX_NO_TEMPORAL") ? 0 : 1;
+    // steps per launch: 2 in FP64, 3 with FP32 storage (measured fastest at
+    // 16384^2); LBMX_TEMPORAL_DEPTH=2 or 3 overrides
+    static int depth_env = -1;
+    if (depth_env < 0)
+        depth_env = getenv("LBMX_TEMPORAL_DEPTH") ? atoi(getenv("LBMX_TEMPORAL_DEPTH")) : 0;
+    const int depth = depth_env ? depth_env : (f64 ? 2 : 3);
     if (L->pull && temporal) {
-        // pairs of steps while at least three remain: the run ends with pull_step
+        // multi-step launches while more steps remain than they take: the run ends
+        // with pull_step
+        if (depth >= 3)
+            for (; nsteps - i >= 4; i += 3) {
+                int rc = f64 ? pull_step3<double>(L) : pull_step3<float>(L);
+                if (rc) return rc;
+            }
         for (; nsteps - i >= 3; i += 2) {
             int rc = f64 ? pull_step2<double>(L) : pull_step2<float>(L);
             if (rc) return rc;

@@ -197,6 +197,74 @@ __global__ void __launch_bounds__(kPull2Threads) k_pull2(const T *__restrict__ S
     }
 }
 
+// Three steps per launch: step 1 fills LDS with the tile and a two-node ring, step 2
+// collides the tile and a one-node ring from it into registers, which replace the
+// LDS contents once every lane has read them, and step 3 is k_pull2's second step.
+// LDS holds 9 x 68 x (TY + 4) values.
+template <typename T, bool W, int TY>
+__global__ void __launch_bounds__(kPull2Threads) k_pull3(const T *__restrict__ S,
+                                                        T *__restrict__ D, Pull2Args a)
+{
+    constexpr int AX = kPull2TX + 4, AN = AX * (TY + 4);
+    constexpr int BX = kPull2TX + 2, BN = BX * (TY + 2);
+    constexpr int NB = (BN + kPull2Threads - 1) / kPull2Threads;
+    __shared__ T buf[Q * AN];
+    const Geo &g = a.g;
+    const int x0 = (blockIdx.x % a.tiles_x) * kPull2TX;
+    const int y0 = (blockIdx.x / a.tiles_x) * TY;
+
+    // step 1: tile + 2-node ring from global memory
+    for (int r = threadIdx.x; r < AN; r += kPull2Threads) {
+        int x = x0 - 2 + r % AX, y = y0 - 2 + r / AX;
+        x = (x % g.Nx + g.Nx) % g.Nx;
+        y = (y % g.Ny + g.Ny) % g.Ny;
+        T p[Q];
+        gather_node(S, g, x, y, p);
+        pull_collide<T, W>(p, g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
+#pragma unroll
+        for (int k = 0; k < Q; k++) buf[k * AN + r] = p[k];
+    }
+    __syncthreads();
+
+    // step 2: tile + 1-node ring, from LDS into registers
+    T pb[NB][Q];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        const int r = threadIdx.x + j * kPull2Threads;
+        if (r < BN) {
+            const int bx = r % BX, by = r / BX;   // A coordinates: (bx + 1, by + 1)
+#pragma unroll
+            for (int k = 0; k < Q; k++)
+                pb[j][k] = buf[k * AN + (by + 1 - D2Q9::cy[k]) * AX + (bx + 1 - D2Q9::cx[k])];
+            pull_collide<T, W>(pb[j], g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        const int r = threadIdx.x + j * kPull2Threads;
+        if (r < BN)
+#pragma unroll
+            for (int k = 0; k < Q; k++) buf[k * BN + r] = pb[j][k];
+    }
+    __syncthreads();
+
+    // step 3: the tile, from LDS
+    for (int q = threadIdx.x; q < kPull2TX * TY; q += kPull2Threads) {
+        const int tx = q % kPull2TX, ty = q / kPull2TX;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= g.Nx || y >= g.Ny) continue;
+        T p[Q];
+#pragma unroll
+        for (int k = 0; k < Q; k++)
+            p[k] = buf[k * BN + (ty + 1 - D2Q9::cy[k]) * BX + (tx + 1 - D2Q9::cx[k])];
+        pull_collide<T, W>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
+        const size_t n = (size_t)y * g.pitch + x;
+#pragma unroll
+        for (int k = 0; k < Q; k++) store_nt(&D[k * g.plane + n], p[k]);
+    }
+}
+
 // f_k = gather(f*_{k-1}) (streaming only).
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather(const T *__restrict__ S, T *__restrict__ D, Geo g)
