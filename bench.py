@@ -6,19 +6,21 @@ the channel/cylinder configuration (1024^2 .. 16384^2).  The lattice is
 initialised on the device; W untimed steps, then K timed steps between two
 stream synchronisations.
 
-Multi-step runs of this lattice take two time steps per launch (k_pull2, the
-gather step with a one-node ring kept in LDS between the steps), bit-identical to
-one launch per step; LBMX_NO_TEMPORAL=1 switches that off.
+Multi-step runs of this lattice take four time steps per launch (k_stream: each
+workgroup walks down a column strip with one wavefront per time level, the
+levels' rows kept in LDS), bit-identical to one launch per step;
+LBMX_NO_TEMPORAL=1 switches that off.
 
 Also reported in the same JSON line:
   roofline      algorithmic bytes of one launch of the dominant kernel (144 B per
                 cell-update in FP64: 9 populations read + 9 written, times the
                 cell-updates of the launch) / its duration, from HIP events on the
-                lattice's stream (two-step launches: the difference of two runs of
-                odd length), vs the 8 TB/s HBM peak; ``traffic`` is the measured HBM
-                bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-                committed under profiles/.  A two-step launch moves less than the
-                per-step algorithmic bytes, so its fraction can exceed 1;
+                lattice's stream (multi-step launches: the difference of two runs
+                of D*m + 1 steps), vs the 8 TB/s HBM peak; ``traffic`` is the
+                measured HBM bytes per launch from the rocprofv3 FETCH_SIZE /
+                WRITE_SIZE summary committed under profiles/.  A D-step launch
+                moves less than D times the per-step algorithmic bytes, so its
+                fraction can exceed 1;
                 ``secondary.headline_kernel`` has the measured bandwidth
   cpu_baseline  the reference library (oracle/_ref, compiled from lbc's own
                 sources with its flags) timed on this host through main_lbm on a
@@ -76,6 +78,15 @@ def time_lattice(lat, steps, warmup):
 
 
 INFINITY_CACHE_BYTES = 256 << 20
+
+# the FP64 gather-form kernel a long run of the periodic lattice launches, by steps
+# per launch (lbmx_launch_depth)
+FP64_KERNELS = {
+    1: "k_pull<double, false>",
+    2: "k_pull2<double, false, 8> (two steps per launch, 64 x 8 tiles)",
+    3: "k_pull3<double, false, 8> (three steps per launch, 64 x 8 tiles)",
+    4: "k_stream<double, false, 4, 64, 4> (four steps per launch, row streaming)",
+}
 
 
 def roofline_sweep(sizes=(1024, 2048, 4096, 8192, 16384), seconds=2.0):
@@ -549,22 +560,23 @@ def main():
 
     lat = tgv_lattice(N, native.F64)
     wall, dev_ms = time_lattice(lat, args.steps, args.warmup)
-    # the dominant kernel and its launch time: a run of n steps is (n - 1) // 2
-    # two-step launches (k_pull2) and one or two one-step launches (k_pull), so the
-    # difference of two odd-length runs is a whole number of k_pull2 launches
-    temporal = not os.environ.get("LBMX_NO_TEMPORAL")
+    # the dominant kernel and its launch time: a run of D*m + 1 steps is m D-step
+    # launches and one one-step launch (k_pull), so the difference of two such runs
+    # is a whole number of D-step launches
+    steps_per_launch = lat.launch_depth()
+    temporal = steps_per_launch > 1
     if temporal:
-        a, b = 61, 21
-        t_a, t_b = lat.time_steps(a), lat.time_steps(b)
-        launch_ms = (t_a - t_b) / ((a - b) // 2)
-        steps_per_launch, kernel = 2, "k_pull2<double, false, 8> (two steps per launch)"
+        m_a, m_b = 40, 10
+        t_a = lat.time_steps(steps_per_launch * m_a + 1)
+        t_b = lat.time_steps(steps_per_launch * m_b + 1)
+        launch_ms = (t_a - t_b) / (m_a - m_b)
     else:
         launch_ms = dev_ms / args.steps
-        steps_per_launch, kernel = 1, "k_pull<double, false>"
+    kernel = FP64_KERNELS.get(steps_per_launch, f"{steps_per_launch} steps per launch")
     lat.close()
     mlups = cells * args.steps / wall / 1e6
     achieved = 144.0 * cells * steps_per_launch / (launch_ms * 1e-3) / 1e9
-    traffic = load_traffic(f"tgv{N}_fp64" + ("_2step" if temporal else ""))
+    traffic = load_traffic(f"tgv{N}_fp64" + (f"_{steps_per_launch}step" if temporal else ""))
     log(f"FP64 {N}^2: {wall / args.steps * 1e3:.3f} ms/step wall, "
         f"{dev_ms / args.steps:.3f} ms/step events, {mlups:.0f} MLUPS; {kernel}: "
         f"{launch_ms:.3f} ms per launch, {achieved:.0f} GB/s algorithmic (144 B per "
@@ -574,22 +586,26 @@ def main():
     if not args.no_secondary:
         lat = tgv_lattice(N, native.F32_SHIFTED)
         w32, d32 = time_lattice(lat, args.steps, args.warmup)
+        spl32 = lat.launch_depth()
         lat.close()
         m32 = cells * args.steps / w32 / 1e6
         secondary["fp32_shifted_tgv"] = {
             "workload": f"taylor_green_periodic_{N}x{N}_fp32_storage_fp32_collision",
             "MLUPS": round(m32, 1),
             "alg_GB_per_s": round(72.0 * cells / (d32 / args.steps * 1e-3) / 1e9, 1),
+            "steps_per_launch": spl32,
             "vs_fp64": round(m32 / mlups, 3)}
         log(f"FP32-shifted {N}^2: {m32:.0f} MLUPS ({m32 / mlups:.2f}x FP64)")
         lat = tgv_lattice(N, native.F32_WIDE)
         ww, dw = time_lattice(lat, args.steps, args.warmup)
+        splw = lat.launch_depth()
         lat.close()
         mw = cells * args.steps / ww / 1e6
         secondary["fp32_wide_tgv"] = {
             "workload": f"taylor_green_periodic_{N}x{N}_fp32_storage_fp64_collision",
             "MLUPS": round(mw, 1),
             "alg_GB_per_s": round(72.0 * cells / (dw / args.steps * 1e-3) / 1e9, 1),
+            "steps_per_launch": splw,
             "vs_fp64": round(mw / mlups, 3)}
         log(f"FP32 storage, FP64 collision {N}^2: {mw:.0f} MLUPS ({mw / mlups:.2f}x FP64)")
         copy = copy_ceiling()
@@ -626,8 +642,9 @@ def main():
         "algorithmic_bytes_per_launch": 144 * cells * steps_per_launch,
         "hbm_bytes_per_launch": traffic,
         "hbm_GB_per_s": round(traffic / (launch_ms * 1e-3) / 1e9, 1) if traffic else None,
-        "note": "two steps per HBM pass move less than the per-step algorithmic bytes, "
-                "so the per-step roofline fraction can exceed 1" if temporal else ""}
+        "note": f"{steps_per_launch} steps per HBM pass move less than the per-step "
+                "algorithmic bytes, so the per-step roofline fraction can exceed 1"
+                if temporal else ""}
     out = {
         "metric": "MLUPS (FP64 fused D2Q9 BGK step) + % HBM roofline",
         "value": round(mlups, 1),

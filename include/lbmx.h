@@ -132,6 +132,29 @@ LBMX_API int lbmx_init_taylor_green(lbmx_lattice *lat, double u0, double kx, dou
  * swap.  Asynchronous on the lattice's stream. */
 LBMX_API int lbmx_step(lbmx_lattice *lat, int nsteps);
 
+/* Launch grouping of multi-step runs on lattices periodic in x and y with no
+ * solid node and no other boundary (no counterpart in the reference; every mode
+ * gives bit-identical results).  Such runs keep post-collision populations and
+ * advance several steps per launch:
+ *   LBMX_LAUNCH_AUTO    row streaming (LBMX_LAUNCH_STREAM) when the lattice gives
+ *                       it at least 512 workgroups, else tiles (the default)
+ *   LBMX_LAUNCH_SINGLE  one step per launch (also the default when the
+ *                       environment sets LBMX_NO_TEMPORAL)
+ *   LBMX_LAUNCH_TILES   two (FP64) or three (FP32 storage) steps per launch on
+ *                       64 x TY tiles with their rings in LDS
+ *   LBMX_LAUNCH_STREAM  4 (FP64, FP32 storage with FP64 collision) or 8 (FP32)
+ *                       steps per launch, each workgroup streaming down a column
+ *                       strip with one wavefront per time level
+ * Every run ends with a one-step launch.  Other lattices ignore the mode. */
+#define LBMX_LAUNCH_AUTO 0
+#define LBMX_LAUNCH_SINGLE 1
+#define LBMX_LAUNCH_TILES 2
+#define LBMX_LAUNCH_STREAM 3
+LBMX_API int lbmx_set_launch(lbmx_lattice *lat, int mode);
+/* Steps per launch of a long run of this lattice under its mode (1 when the
+ * lattice does not qualify). */
+LBMX_API int lbmx_launch_depth(const lbmx_lattice *lat);
+
 /* Time-domain impedance boundary (TDIBC) driven on the device every step, the
  * scheme of cases/nit/nit.c:13-47 and :275-303: before each step the wall
  * pressure p' = cs*cs*(rho - rho0)*cp is taken from the lagged density of the

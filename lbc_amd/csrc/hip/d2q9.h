@@ -12,6 +12,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <utility>
 
 namespace lbmx {
 
@@ -41,20 +42,40 @@ struct Moments {
     double rho, ux, uy;
 };
 
+// c.a for a lattice vector c with components in {-1, 0, 1}: a sum, a difference
+// or a signed copy.  The reference multiplies by the components as doubles; a
+// product with +-1 is exact and one with 0 only adds a signed zero, so for finite
+// values these agree with it bit for bit up to the sign of an exact zero.
+template <int CX, int CY, typename T>
+__device__ __forceinline__ T cdot(T ax, T ay)
+{
+    if constexpr (CX == 0) return CY > 0 ? ay : -ay;
+    else if constexpr (CY == 0) return CX > 0 ? ax : -ax;
+    else if constexpr (CX > 0) return CY > 0 ? ax + ay : ax - ay;
+    else return CY > 0 ? ay - ax : -ax - ay;
+}
+
+// compute_macroscopic_fields.c:27-41 (and collision.c's lagged moments): the
+// sums run over k = 0..8 in order; the terms with a zero velocity component are
+// left out of sx and sy (they only add signed zeros).
 __device__ __forceinline__ Moments moments_ref(const double f[Q], double Fx, double Fy)
 {
-    double s = 0.0, sx = 0.0, sy = 0.0;
+    static_assert(D2Q9::cx[1] == 1 && D2Q9::cx[3] == -1 && D2Q9::cx[5] == 1 &&
+                      D2Q9::cx[6] == -1 && D2Q9::cx[7] == -1 && D2Q9::cx[8] == 1 &&
+                      D2Q9::cy[2] == 1 && D2Q9::cy[4] == -1 && D2Q9::cy[5] == 1 &&
+                      D2Q9::cy[6] == 1 && D2Q9::cy[7] == -1 && D2Q9::cy[8] == -1,
+                  "D2Q9 ordering of nit.c:195-202");
+    double s = f[0];
 #pragma unroll
-    for (int k = 0; k < Q; k++) {
-        s += f[k];
-        sx += f[k] * (double)D2Q9::cx[k];
-        sy += f[k] * (double)D2Q9::cy[k];
-    }
-    if (s < kRhoFloor) s = kRhoFloor;
+    for (int k = 1; k < Q; k++) s += f[k];
+    const double sx = ((((f[1] - f[3]) + f[5]) - f[6]) - f[7]) + f[8];
+    const double sy = ((((f[2] - f[4]) + f[5]) + f[6]) - f[7]) - f[8];
+    double r = s;
+    if (r < kRhoFloor) r = kRhoFloor;
     Moments m;
-    m.rho = s;
-    m.ux = (sx + 0.5 * Fx) / s;
-    m.uy = (sy + 0.5 * Fy) / s;
+    m.rho = r;
+    m.ux = (sx + 0.5 * Fx) / r;
+    m.uy = (sy + 0.5 * Fy) / r;
     return m;
 }
 
@@ -67,23 +88,46 @@ __device__ __forceinline__ double feq_ref(double w, double cxk, double cyk, doub
     return w * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
 }
 
-// Post-collision value of one population (collision.c:30-37).  With a zero
-// body force the Guo term is +-0 and is skipped: f + (x + -0) == f + x for
-// every non-zero x, and the sign of an exact zero population is irrelevant.
-__device__ __forceinline__ double collide_one(double f, double w, double cxk, double cyk,
-                                              double rho, double ux, double uy, double usq,
-                                              double om, bool force, double Fx, double Fy,
-                                              double uF)
+// Post-collision value of population K (collision.c:30-37), the reference's
+// expression with c.u and c.F formed by cdot (exact for the +-1 components; the
+// zero components only add signed zeros, so for K = 0 the equilibrium is
+// w rho (1 - 1.5 u.u) exactly).  With a zero body force the Guo term is +-0 and
+// is skipped: f + (x + -0) == f + x for every non-zero x, and the sign of an
+// exact zero population is irrelevant.
+template <int K>
+__device__ __forceinline__ double collide_one(double f, double rho, double ux, double uy,
+                                              double usq, double om, bool force, double Fx,
+                                              double Fy, double uF)
 {
-    double cu = cxk * ux + cyk * uy;
-    double eq = w * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+    constexpr int cx = D2Q9::cx[K], cy = D2Q9::cy[K];
+    const double w = D2Q9::w[K];
+    double cu = 0.0, eq;
+    if constexpr (K == 0) {
+        eq = w * rho * (1.0 - 1.5 * usq);
+    } else {
+        cu = cdot<cx, cy>(ux, uy);
+        eq = w * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+    }
     double r = -om * (f - eq);
     if (force) {
-        double cF = cxk * Fx + cyk * Fy;
-        double Fk = w * (3.0 * cF + 9.0 * cu * cF - 3.0 * uF);
+        double Fk;
+        if constexpr (K == 0) {
+            Fk = w * -(3.0 * uF);
+        } else {
+            const double cF = cdot<cx, cy>(Fx, Fy);
+            Fk = w * (3.0 * cF + 9.0 * cu * cF - 3.0 * uF);
+        }
         r = r + (1 - 0.5 * om) * Fk;
     }
     return f + r;
+}
+
+template <int... Ks>
+__device__ __forceinline__ void collide_all(double f[Q], const Moments &m, double usq, double om,
+                                            bool force, double Fx, double Fy, double uF,
+                                            std::integer_sequence<int, Ks...>)
+{
+    ((f[Ks] = collide_one<Ks>(f[Ks], m.rho, m.ux, m.uy, usq, om, force, Fx, Fy, uF)), ...);
 }
 
 __device__ __forceinline__ void collide_ref(double f[Q], const Moments &m, double om,
@@ -91,10 +135,7 @@ __device__ __forceinline__ void collide_ref(double f[Q], const Moments &m, doubl
 {
     double usq = m.ux * m.ux + m.uy * m.uy;
     double uF = m.ux * Fx + m.uy * Fy;
-#pragma unroll
-    for (int k = 0; k < Q; k++)
-        f[k] = collide_one(f[k], D2Q9::w[k], (double)D2Q9::cx[k], (double)D2Q9::cy[k], m.rho,
-                           m.ux, m.uy, usq, om, force, Fx, Fy, uF);
+    collide_all(f, m, usq, om, force, Fx, Fy, uF, std::make_integer_sequence<int, Q>{});
 }
 
 }  // namespace lbmx

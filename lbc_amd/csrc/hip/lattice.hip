@@ -187,6 +187,7 @@ extern "C" int lbmx_create(const lbmx_config *cfg, lbmx_lattice **out)
 
     lbmx_lattice *L = new lbmx_lattice();
     L->cfg = *cfg;
+    L->launch_mode = getenv("LBMX_NO_TEMPORAL") ? LBMX_LAUNCH_SINGLE : LBMX_LAUNCH_AUTO;
     if (L->cfg.precision != LBMX_F64 && L->cfg.rho0 == 0.0) L->cfg.rho0 = 1.0;
     if (L->cfg.precision == LBMX_F64) L->cfg.rho0 = 0.0;
     L->Nx = cfg->Nx;
@@ -1225,6 +1226,86 @@ static void launch_pull3(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
                        D, a);
 }
 
+// ---- row streaming (k_stream): D steps per launch ---------------------------------
+// Depth, prefetch and rows per workgroup were measured on the periodic Taylor-Green
+// lattice at 1024^2 .. 16384^2: FP64 4 steps (LDS 53.6 KB, three workgroups per
+// CU), FP32 storage 8 steps with FP32 collision (two workgroups of 8 wavefronts)
+// and 4 with FP64 collision.
+constexpr int kStreamLW = 64;
+
+static int stream_depth(const lbmx_lattice *L)
+{
+    return (L->esize == 4 && L->cfg.precision != LBMX_F32_WIDE) ? 8 : 4;
+}
+
+// Rows per workgroup: about 4096 workgroups over the lattice, as a power of two
+// between 32 and 128 at depth 4, and between 128 (the depth-8 pipeline takes 21
+// iterations to fill) and 512 at depth 8.
+static int stream_rows(const lbmx_lattice *L, int depth)
+{
+    const int tx = kStreamLW - 2 * (depth - 1);
+    const double strips = (double)((L->Nx + tx - 1) / tx);
+    const double want = strips * L->Ny / 4096.0;
+    const int lo = depth <= 4 ? 32 : 128, hi = depth <= 4 ? 128 : 512;
+    int rows = lo;
+    while (rows < hi && want >= rows * 1.41421356) rows *= 2;
+    return std::min(rows, L->Ny);
+}
+
+// k_stream pays off once it has at least 512 workgroups (two per CU); smaller
+// lattices run the tile kernels.
+static bool stream_fits(const lbmx_lattice *L, int depth)
+{
+    const int tx = kStreamLW - 2 * (depth - 1);
+    const int rows = stream_rows(L, depth);
+    const long long wgs = (long long)((L->Nx + tx - 1) / tx) * ((L->Ny + rows - 1) / rows);
+    return wgs >= 512;
+}
+
+template <typename T, bool W, int D, int P>
+static void launch_stream(lbmx_lattice *L, const T *S, T *Dst, StreamArgs &a)
+{
+    constexpr int TX = StreamShape<D, kStreamLW>::TX;
+    a.strips = (L->Nx + TX - 1) / TX;
+    a.seg_rows = stream_rows(L, D);
+    a.segs = (L->Ny + a.seg_rows - 1) / a.seg_rows;
+    const unsigned blocks = ((unsigned)a.strips * (unsigned)a.segs + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_stream<T, W, D, kStreamLW, P>), dim3(blocks), dim3(kStreamLW * D), 0,
+                       L->stream, S, Dst, a);
+}
+
+// stream_depth(L) gather-form steps in one launch; as pull_step2 otherwise.
+template <typename T>
+static int pull_stream(lbmx_lattice *L)
+{
+    const Geo g = lbmx_geo(L);
+    StepFlags s = lbmx_flags(L);
+    StreamArgs a;
+    a.g = g;
+    a.force = s.force;
+    a.Fx = s.Fx;
+    a.Fy = s.Fy;
+    a.Fx_lag = s.Fx_lag;
+    a.Fy_lag = s.Fy_lag;
+    const T *S = (const T *)L->pop[L->pstar];
+    T *D = (T *)L->pop[L->pstar ^ 1];
+    if constexpr (sizeof(T) == 8)
+        launch_stream<T, false, 4, 4>(L, S, D, a);
+    else if (g.wide)
+        launch_stream<T, true, 4, 4>(L, S, D, a);
+    else
+        launch_stream<T, false, 8, 6>(L, S, D, a);
+    HIPCHK(hipGetLastError());
+    L->pstar ^= 1;
+    s.Fx_lag = L->cfg.Fx;
+    s.Fy_lag = L->cfg.Fy;
+    L->last = s;
+    L->Fx_lag = L->cfg.Fx;
+    L->Fy_lag = L->cfg.Fy;
+    L->steps += stream_depth(L);
+    return LBMX_OK;
+}
+
 // Three gather-form steps in one launch (k_pull3); as pull_step2 otherwise.
 template <typename T>
 static int pull_step3(lbmx_lattice *L)
@@ -1316,6 +1397,29 @@ int lbmx_ensure_push(lbmx_lattice *L)
     return LBMX_OK;
 }
 
+extern "C" int lbmx_set_launch(lbmx_lattice *L, int mode)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (mode < LBMX_LAUNCH_AUTO || mode > LBMX_LAUNCH_STREAM)
+        return lbmx_fail(LBMX_ERR_ARG, "unknown launch mode %d", mode);
+    L->launch_mode = mode;
+    return LBMX_OK;
+}
+
+extern "C" int lbmx_launch_depth(const lbmx_lattice *L)
+{
+    if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!pull_eligible(L) || L->launch_mode == LBMX_LAUNCH_SINGLE) return 1;
+    const int sd = stream_depth(L);
+    if (L->launch_mode == LBMX_LAUNCH_STREAM ||
+        (L->launch_mode == LBMX_LAUNCH_AUTO && stream_fits(L, sd)))
+        return sd;
+    static int depth_env = -1;
+    if (depth_env < 0)
+        depth_env = getenv("LBMX_TEMPORAL_DEPTH") ? atoi(getenv("LBMX_TEMPORAL_DEPTH")) : 0;
+    return depth_env ? depth_env : (L->esize == 8 ? 2 : 3);
+}
+
 extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
@@ -1334,17 +1438,22 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
         if (rc) return rc;
     }
     int i = 0;
-    static int temporal = -1;   // LBMX_NO_TEMPORAL=1: one launch per step in gather form too
-    if (temporal < 0) temporal = getenv("LBMX_NO_TEMPORAL") ? 0 : 1;
-    // steps per launch: 2 in FP64, 3 with FP32 storage (measured fastest at
-    // 16384^2); LBMX_TEMPORAL_DEPTH=2 or 3 overrides
+    // tile depth: 2 steps per launch in FP64, 3 with FP32 storage (measured fastest
+    // at 16384^2); LBMX_TEMPORAL_DEPTH=2 or 3 overrides
     static int depth_env = -1;
     if (depth_env < 0)
         depth_env = getenv("LBMX_TEMPORAL_DEPTH") ? atoi(getenv("LBMX_TEMPORAL_DEPTH")) : 0;
     const int depth = depth_env ? depth_env : (f64 ? 2 : 3);
-    if (L->pull && temporal) {
+    const int mode = L->launch_mode;
+    if (L->pull && mode != LBMX_LAUNCH_SINGLE) {
         // multi-step launches while more steps remain than they take: the run ends
         // with pull_step
+        const int sd = stream_depth(L);
+        if (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && stream_fits(L, sd)))
+            for (; nsteps - i >= sd + 1; i += sd) {
+                int rc = f64 ? pull_stream<double>(L) : pull_stream<float>(L);
+                if (rc) return rc;
+            }
         if (depth >= 3)
             for (; nsteps - i >= 4; i += 3) {
                 int rc = f64 ? pull_step3<double>(L) : pull_step3<float>(L);

@@ -33,6 +33,7 @@ struct lbmx_lattice {
     hipStream_t stream = nullptr;
     void *pop[2] = {nullptr, nullptr};
     int cur = 0;           // buffer holding the state the next step starts from
+    int launch_mode = 0;   // LBMX_LAUNCH_* (multi-step launches of the gather form)
     double *R = nullptr, *U = nullptr, *V = nullptr;
 
     // geometry

@@ -17,6 +17,7 @@
 //                    buffer that held f*_k; the other buffer keeps f*_{k-1},
 //                    which is exactly the reference's f_new after the swap.
 #pragma once
+#include <utility>
 #include "step_kernels.h"
 
 namespace lbmx {
@@ -262,6 +263,170 @@ __global__ void __launch_bounds__(kPull2Threads) k_pull3(const T *__restrict__ S
         const size_t n = (size_t)y * g.pitch + x;
 #pragma unroll
         for (int k = 0; k < Q; k++) store_nt(&D[k * g.plane + n], p[k]);
+    }
+}
+
+// ---- D steps per launch by row streaming -------------------------------------
+// A workgroup owns a strip of TX = LW - 2(D-1) columns and a segment of rows, and
+// walks down the segment one row per iteration.  Time level l (1..D) is computed
+// by its own LW/64 wavefronts on a row of LW - 2(l-1) columns (the strip plus
+// the ring level l still needs); level 1 gathers from S in global memory, levels
+// 2..D gather from level l-1's rows in LDS, and level D stores f*_{s+D} to D.
+// Level l works two rows behind level l-1, so everything an iteration reads was
+// written in an earlier one: one barrier per iteration, and each level keeps its
+// last four rows in an LDS ring.  Level 1 keeps P rows of loads in flight in
+// registers.  Compared with the tiles of k_pull2/k_pull3, no row is computed
+// twice (only the 2(D-1) ring columns of a strip are), and the loads of one
+// wavefront overlap the arithmetic of the others.  Same arithmetic on the same
+// values as D k_pull launches: bit-identical.
+struct StreamArgs {
+    Geo g;
+    int force;
+    double Fx, Fy;            // body force of every step
+    double Fx_lag, Fy_lag;    // force the first step's lagged moments were formed with
+    int strips;               // strips per row of the lattice
+    int seg_rows;             // output rows per workgroup
+    int segs;                 // row segments
+};
+
+// gather_node for a wavefront-uniform row y: the row and plane offsets stay in
+// scalar registers and each lane adds one of three fixed 32-bit byte offsets (its
+// column x - 1, x, x + 1, wrapped), so the loads need no per-lane address math.
+template <typename T>
+__device__ __forceinline__ void gather_row(const T *__restrict__ S, const Geo &g, int y,
+                                           unsigned bxm, unsigned bx, unsigned bxp, T f[Q])
+{
+    const int ym = (y == 0) ? g.Ny - 1 : y - 1, yp = (y == g.Ny - 1) ? 0 : y + 1;
+    const char *r0 = reinterpret_cast<const char *>(S + (size_t)y * g.pitch);
+    const char *rm = reinterpret_cast<const char *>(S + (size_t)ym * g.pitch);
+    const char *rp = reinterpret_cast<const char *>(S + (size_t)yp * g.pitch);
+#pragma unroll
+    for (int k = 0; k < Q; k++) {
+        const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
+        const char *base = (cy == 1 ? rm : (cy == -1 ? rp : r0)) + (size_t)k * g.plane * sizeof(T);
+        const unsigned off = cx == 1 ? bxm : (cx == -1 ? bxp : bx);
+        f[k] = *reinterpret_cast<const T *>(base + off);
+    }
+}
+
+template <int D, int LW>
+struct StreamShape {
+    static constexpr int TX = LW - 2 * (D - 1);                  // output columns
+    static constexpr int width(int l) { return LW - 2 * (l - 1); }
+    static constexpr int offset(int l)                           // LDS values before level l
+    {
+        int o = 0;
+        for (int m = 1; m < l; m++) o += 4 * Q * width(m);
+        return o;
+    }
+    static constexpr int lds = offset(D);                        // levels 1..D-1
+};
+
+template <typename T, bool W, int D, int LW, int L>
+__device__ __forceinline__ void stream_level(T *buf, const T *__restrict__ S, T *__restrict__ Dst,
+                                             const StreamArgs &a, int lane, int x0, int y0, int y1,
+                                             int iters)
+{
+    using Sh = StreamShape<D, LW>;
+    constexpr int Win = Sh::width(L - 1), Wout = Sh::width(L);
+    const Geo &g = a.g;
+    const int rbase = y0 - (D - 1);
+    const T *in = buf + Sh::offset(L - 1);
+    for (int t = 0; t < iters; t++) {
+        const int r = rbase + t - 2 * (L - 1);
+        const bool active = r >= y0 - (D - L) && r <= y1 - 1 + (D - L);
+        if (active && lane < Wout) {
+            T p[Q];
+#pragma unroll
+            for (int k = 0; k < Q; k++) {
+                const int s = (r - D2Q9::cy[k] - rbase + 4 * D) & 3;
+                p[k] = in[(s * Q + k) * Win + lane + 1 - D2Q9::cx[k]];
+            }
+            pull_collide<T, W>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
+            if constexpr (L < D) {
+                T *out = buf + Sh::offset(L) + ((r - rbase + 4 * D) & 3) * Q * Wout + lane;
+#pragma unroll
+                for (int k = 0; k < Q; k++) out[k * Wout] = p[k];
+            } else {
+                const int x = x0 + lane;
+                if (x < g.Nx) {
+                    char *row = reinterpret_cast<char *>(Dst + (size_t)r * g.pitch);
+                    const unsigned bx = (unsigned)x * (unsigned)sizeof(T);
+#pragma unroll
+                    for (int k = 0; k < Q; k++)
+                        store_nt(reinterpret_cast<T *>(row + (size_t)k * g.plane * sizeof(T) + bx),
+                                 p[k]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, bool W, int D, int LW, int... Ls>
+__device__ __forceinline__ void stream_levels(int level, T *buf, const T *__restrict__ S,
+                                              T *__restrict__ Dst, const StreamArgs &a, int lane,
+                                              int x0, int y0, int y1, int iters,
+                                              std::integer_sequence<int, Ls...>)
+{
+    ((level == Ls + 2 ? stream_level<T, W, D, LW, Ls + 2>(buf, S, Dst, a, lane, x0, y0, y1, iters)
+                      : void()),
+     ...);
+}
+
+template <typename T, bool W, int D, int LW, int P>
+__global__ void __launch_bounds__(LW * D) k_stream(const T *__restrict__ S, T *__restrict__ Dst,
+                                                  StreamArgs a)
+{
+    using Sh = StreamShape<D, LW>;
+    static_assert(LW % 64 == 0 && Sh::TX > 0, "strip shape");
+    __shared__ T buf[Sh::lds];
+    const Geo &g = a.g;
+    const int level = __builtin_amdgcn_readfirstlane(threadIdx.x / LW) + 1;
+    const int lane = threadIdx.x % LW;
+    // XCD-aware order: consecutive workgroups go to the 8 XCDs in turn, so XCD i
+    // takes the i-th eighth of the strip-major work list (strips of one segment,
+    // then the next segment) and neighbouring strips share that XCD's L2
+    const int work = a.strips * a.segs, per = (work + 7) / 8;
+    const int item = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (item >= work) return;   // whole workgroup, before any barrier
+    const int y0 = (item / a.strips) * a.seg_rows;
+    const int y1 = min(y0 + a.seg_rows, g.Ny);
+    const int x0 = (item % a.strips) * Sh::TX;
+    // every wavefront runs the same number of iterations (a multiple of P), one
+    // barrier each; iterations past a level's rows do nothing
+    const int iters = ((y1 - y0) + 3 * (D - 1) + P - 1) / P * P;
+    const int rbase = y0 - (D - 1);
+
+    if (level == 1) {
+        // the loads are unconditional (rows past the last are clamped to it), so the
+        // compiler's wait counts let P rows of loads stay in flight
+        const int n1 = (y1 - y0) + 2 * (D - 1);   // level-1 rows
+        const int x = ((x0 - (D - 1) + lane) % g.Nx + g.Nx) % g.Nx;
+        const unsigned bx = (unsigned)x * (unsigned)sizeof(T);
+        const unsigned bxm = (unsigned)(x == 0 ? g.Nx - 1 : x - 1) * (unsigned)sizeof(T);
+        const unsigned bxp = (unsigned)(x == g.Nx - 1 ? 0 : x + 1) * (unsigned)sizeof(T);
+        auto row = [&](int t) { return ((rbase + min(t, n1 - 1)) % g.Ny + g.Ny) % g.Ny; };
+        T pre[P][Q];
+#pragma unroll
+        for (int j = 0; j < P; j++) gather_row(S, g, row(j), bxm, bx, bxp, pre[j]);
+        for (int t0 = 0; t0 < iters; t0 += P) {
+#pragma unroll
+            for (int j = 0; j < P; j++) {
+                const int t = t0 + j;
+                if (t < n1) {
+                    pull_collide<T, W>(pre[j], g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
+                    T *out = buf + ((t + 4 * D) & 3) * Q * LW + lane;
+#pragma unroll
+                    for (int k = 0; k < Q; k++) out[k * LW] = pre[j][k];
+                }
+                gather_row(S, g, row(t + P), bxm, bx, bxp, pre[j]);
+                __syncthreads();
+            }
+        }
+    } else {
+        stream_levels<T, W, D, LW>(level, buf, S, Dst, a, lane, x0, y0, y1, iters,
+                                   std::make_integer_sequence<int, D - 1>{});
     }
 }
 

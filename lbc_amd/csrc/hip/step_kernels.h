@@ -36,6 +36,7 @@
 // reference's full state after every step.
 #pragma once
 #include <math.h>
+#include <utility>
 #include "d2q9.h"
 
 namespace lbmx {
@@ -104,21 +105,63 @@ struct MomentsF {
 };
 
 // Explicit fmaf() everywhere a fused multiply-add is wanted (and no contraction
-// pragma), so every kernel that inlines these rounds identically.
+// pragma), so every kernel that inlines these rounds identically.  1/rho is the
+// hardware reciprocal (within 1 ulp); the velocity it scales carries FP32
+// rounding of the same order anyway.
 __device__ __forceinline__ MomentsF moments_f32(const float g[Q], float rho0, float Fx,
                                                 float Fy)
 {
-    float dr = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7])) + g[8];
+    const float s56 = g[5] + g[6], s78 = g[7] + g[8];
+    float dr = (((g[0] + g[1]) + (g[2] + g[3])) + g[4]) + (s56 + s78);
     float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
-    float jy = (g[2] - g[4]) + (g[5] + g[6]) - (g[7] + g[8]);
+    float jy = ((g[2] - g[4]) + s56) - s78;
     float rho = rho0 + dr;
     if (rho < 1e-16f) rho = 1e-16f;
-    float inv = 1.0f / rho;
+    const float inv = __builtin_amdgcn_rcpf(rho);
     MomentsF m;
     m.dr = rho - rho0;
     m.ux = __fmaf_rn(0.5f, Fx, jx) * inv;
     m.uy = __fmaf_rn(0.5f, Fy, jy) * inv;
     return m;
+}
+
+template <int K>
+__device__ __forceinline__ float collide_f32_one(float gk, const MomentsF &m, float rho,
+                                                 float usq15, float om, bool force, float Fx,
+                                                 float Fy, float uF3, float gf)
+{
+    constexpr int cx = D2Q9::cx[K], cy = D2Q9::cy[K];
+    const float w = (float)D2Q9::w[K];
+    float geq;
+    float cu = 0.0f;
+    if constexpr (K == 0) {
+        geq = w * __fmaf_rn(rho, -usq15, m.dr);
+    } else {
+        cu = cdot<cx, cy>(m.ux, m.uy);
+        const float t = __fmaf_rn(cu, __fmaf_rn(4.5f, cu, 3.0f), -usq15);
+        geq = w * __fmaf_rn(rho, t, m.dr);
+    }
+    float r = __fmaf_rn(-om, gk - geq, gk);
+    if (force) {
+        float s;
+        if constexpr (K == 0) {
+            s = -uF3;
+        } else {
+            const float cF = cdot<cx, cy>(Fx, Fy);
+            s = __fmaf_rn(9.0f * cu, cF, __fmaf_rn(3.0f, cF, -uF3));
+        }
+        r = __fmaf_rn(gf * w, s, r);
+    }
+    return r;
+}
+
+template <int... Ks>
+__device__ __forceinline__ void collide_f32_all(float g[Q], const MomentsF &m, float rho,
+                                                float usq15, float om, bool force, float Fx,
+                                                float Fy, float uF3, float gf,
+                                                std::integer_sequence<int, Ks...>)
+{
+    ((g[Ks] = collide_f32_one<Ks>(g[Ks], m, rho, usq15, om, force, Fx, Fy, uF3, gf)), ...);
 }
 
 __device__ __forceinline__ void collide_f32(float g[Q], const MomentsF &m, float rho0, float om,
@@ -128,24 +171,8 @@ __device__ __forceinline__ void collide_f32(float g[Q], const MomentsF &m, float
     const float usq15 = 1.5f * __fmaf_rn(m.ux, m.ux, m.uy * m.uy);
     const float uF3 = 3.0f * __fmaf_rn(m.ux, Fx, m.uy * Fy);
     const float gf = __fmaf_rn(-0.5f, om, 1.0f);
-#pragma unroll
-    for (int k = 0; k < Q; k++) {
-        const float w = (float)D2Q9::w[k];
-        const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
-        // c.u and c.F with c in {-1,0,1}: exact sums/differences
-        const float cu = (cx == 0 ? 0.0f : (cx > 0 ? m.ux : -m.ux)) +
-                         (cy == 0 ? 0.0f : (cy > 0 ? m.uy : -m.uy));
-        const float t = __fmaf_rn(cu, __fmaf_rn(4.5f, cu, 3.0f), -usq15);
-        const float geq = w * __fmaf_rn(rho, t, m.dr);
-        float r = __fmaf_rn(-om, g[k] - geq, g[k]);
-        if (force) {
-            const float cF = (cx == 0 ? 0.0f : (cx > 0 ? Fx : -Fx)) +
-                             (cy == 0 ? 0.0f : (cy > 0 ? Fy : -Fy));
-            const float s = __fmaf_rn(9.0f * cu, cF, __fmaf_rn(3.0f, cF, -uF3));
-            r = __fmaf_rn(gf * w, s, r);
-        }
-        g[k] = r;
-    }
+    collide_f32_all(g, m, rho, usq15, om, force, Fx, Fy, uF3, gf,
+                    std::make_integer_sequence<int, Q>{});
 }
 
 // ---- FP64 arithmetic on FP32-stored deviations (LBMX_F32_WIDE) ---------------
@@ -170,11 +197,50 @@ __device__ __forceinline__ MomentsW moments_wide(const float g[Q], double rho0, 
     const double jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
     double rho = rho0 + dr;
     if (rho < kRhoFloor) rho = kRhoFloor;
+    const double inv = 1.0 / rho;
     MomentsW m;
     m.dr = rho - rho0;
-    m.ux = (jx + 0.5 * Fx) / rho;
-    m.uy = (jy + 0.5 * Fy) / rho;
+    m.ux = (jx + 0.5 * Fx) * inv;
+    m.uy = (jy + 0.5 * Fy) * inv;
     return m;
+}
+
+template <int K>
+__device__ __forceinline__ float collide_wide_one(float g, const MomentsW &m, double rho,
+                                                  double usq15, double om, bool force, double Fx,
+                                                  double Fy, double uF3, double gf)
+{
+    constexpr int cx = D2Q9::cx[K], cy = D2Q9::cy[K];
+    const double w = D2Q9::w[K];
+    double cu = 0.0, geq;
+    if constexpr (K == 0) {
+        geq = w * (m.dr - rho * usq15);
+    } else {
+        cu = cdot<cx, cy>(m.ux, m.uy);
+        geq = w * (m.dr + rho * (cu * (3.0 + 4.5 * cu) - usq15));
+    }
+    const double gk = (double)g;
+    double r = gk - om * (gk - geq);
+    if (force) {
+        double s;
+        if constexpr (K == 0) {
+            s = -uF3;
+        } else {
+            const double cF = cdot<cx, cy>(Fx, Fy);
+            s = 3.0 * cF + 9.0 * cu * cF - uF3;
+        }
+        r = r + gf * w * s;
+    }
+    return (float)r;
+}
+
+template <int... Ks>
+__device__ __forceinline__ void collide_wide_all(float g[Q], const MomentsW &m, double rho,
+                                                 double usq15, double om, bool force, double Fx,
+                                                 double Fy, double uF3, double gf,
+                                                 std::integer_sequence<int, Ks...>)
+{
+    ((g[Ks] = collide_wide_one<Ks>(g[Ks], m, rho, usq15, om, force, Fx, Fy, uF3, gf)), ...);
 }
 
 __device__ __forceinline__ void collide_wide(float g[Q], const MomentsW &m, double rho0, double om,
@@ -184,22 +250,8 @@ __device__ __forceinline__ void collide_wide(float g[Q], const MomentsW &m, doub
     const double usq15 = 1.5 * (m.ux * m.ux + m.uy * m.uy);
     const double uF3 = 3.0 * (m.ux * Fx + m.uy * Fy);
     const double gf = 1.0 - 0.5 * om;
-#pragma unroll
-    for (int k = 0; k < Q; k++) {
-        const double w = D2Q9::w[k];
-        const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
-        const double cu = (cx == 0 ? 0.0 : (cx > 0 ? m.ux : -m.ux)) +
-                          (cy == 0 ? 0.0 : (cy > 0 ? m.uy : -m.uy));
-        const double geq = w * (m.dr + rho * (cu * (3.0 + 4.5 * cu) - usq15));
-        const double gk = (double)g[k];
-        double r = gk - om * (gk - geq);
-        if (force) {
-            const double cF = (cx == 0 ? 0.0 : (cx > 0 ? Fx : -Fx)) +
-                              (cy == 0 ? 0.0 : (cy > 0 ? Fy : -Fy));
-            r = r + gf * w * (3.0 * cF + 9.0 * cu * cF - uF3);
-        }
-        g[k] = (float)r;
-    }
+    collide_wide_all(g, m, rho, usq15, om, force, Fx, Fy, uF3, gf,
+                     std::make_integer_sequence<int, Q>{});
 }
 
 // Lagged moments in the reference's FP64 form from either FP32 arithmetic.

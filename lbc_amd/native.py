@@ -16,6 +16,8 @@ from . import lib
 F64 = 0
 F32_SHIFTED = 1      # FP32 storage of f - w*rho0, FP32 collision
 F32_WIDE = 2         # the same storage, collision in FP64
+# launch grouping of multi-step runs on periodic all-fluid lattices (lbmx_set_launch)
+LAUNCH_AUTO, LAUNCH_SINGLE, LAUNCH_TILES, LAUNCH_STREAM = 0, 1, 2, 3
 EXACT_STATE = 1
 
 BC_CODES = {
@@ -86,6 +88,8 @@ def _bind():
     L.lbmx_set_uniform.argtypes = [vp, d]
     L.lbmx_set_force.argtypes = [vp, d, d]
     L.lbmx_time_steps.argtypes = [vp, i, _dp]
+    L.lbmx_set_launch.argtypes = [vp, i]
+    L.lbmx_launch_depth.argtypes = [vp]
     L.lbmx_set_tdibc.argtypes = [vp, C.POINTER(Tdibc)]
     L.lbmx_tdibc_history.argtypes = [vp, _dp, i]
     L.lbmx_set_probes.argtypes = [vp, _ip, _ip, i, i]
@@ -229,6 +233,16 @@ class Lattice:
 
     def set_force(self, Fx, Fy):
         check(self._L.lbmx_set_force(self.h, float(Fx), float(Fy)))
+
+    def set_launch(self, mode):
+        """Launch grouping of multi-step runs (LAUNCH_AUTO, _SINGLE, _TILES, _STREAM)."""
+        check(self._L.lbmx_set_launch(self.h, int(mode)))
+
+    def launch_depth(self):
+        """Steps per launch of a long run of this lattice under its launch mode."""
+        r = self._L.lbmx_launch_depth(self.h)
+        check(min(r, 0))
+        return r
 
     def time_steps(self, n):
         """Device milliseconds (HIP events on the lattice's stream) for n steps."""

@@ -130,6 +130,87 @@ def test_gather_form_bit_identical(sc, chunks):
     assert same_bits(rho, ref.rho) and same_bits(u, ref.u) and same_bits(v, ref.v)
 
 
+LAUNCH_CASES = GATHER_CASES + [
+    # several column strips and row segments, the last of each partial
+    Scenario("tgv_strips", 300, 200, [("periodic_x", 0), ("periodic_y", 0)], 34,
+             Fx=2e-6, Fy=1e-6),
+    Scenario("tgv_thin", 3, 150, [("periodic_x", 0), ("periodic_y", 0)], 35),
+    Scenario("tgv_flat", 130, 3, [("periodic_x", 0), ("periodic_y", 0)], 36),
+]
+
+
+def _launch_run(sc, state, chunks, mode, precision=0):
+    from lbc_amd import native
+    lat = native.Lattice(sc.Nx, sc.Ny, precision=precision, per_x=1, per_y=1,
+                         Fx=sc.Fx, Fy=sc.Fy)
+    lat.set_launch(mode)
+    lat.set_geometry(sc.mask, sc.omega, sc.phi)
+    lat.set_boundaries(sc.bc_names)
+    lat.upload(*state)
+    for n in chunks:
+        lat.step(n)
+    rho, u, v = lat.moments()
+    f, fn = lat.download(both=True)
+    lat.close()
+    return f, fn, rho, u, v
+
+
+@pytest.mark.parametrize("sc", LAUNCH_CASES, ids=[s.name for s in LAUNCH_CASES])
+@pytest.mark.parametrize("mode", [1, 2, 3], ids=["single", "tiles", "stream"])
+def test_launch_modes_bit_identical(sc, mode):
+    """One step per launch, tiles (2 steps per launch) and row streaming (4 steps
+    per launch, forced on these small lattices), over runs of 19 and 6 + 11 steps:
+    every buffer and moment equals the oracle bit for bit."""
+    state = sc.initial_state()
+    ref, _ = run_oracle(sc, state, 17)
+    got = _launch_run(sc, state, (6, 11), mode)
+    for a, b in zip(got, (ref.f, ref.f_new, ref.rho, ref.u, ref.v)):
+        assert same_bits(a, b)
+    ref, _ = run_oracle(sc, state, 19)
+    got = _launch_run(sc, state, (19,), mode)
+    for a, b in zip(got, (ref.f, ref.f_new, ref.rho, ref.u, ref.v)):
+        assert same_bits(a, b)
+
+
+@pytest.mark.parametrize("precision", [1, 2], ids=["f32", "f32wide"])
+@pytest.mark.parametrize("sc", LAUNCH_CASES, ids=[s.name for s in LAUNCH_CASES])
+def test_launch_modes_fp32_identical(sc, precision):
+    """FP32 storage: tiles (3 steps per launch) and row streaming (8 steps with
+    FP32 collision, 4 with FP64 collision) round exactly as one step per launch."""
+    state = sc.initial_state()
+    ref = _launch_run(sc, state, (25,), 1, precision)
+    for mode in (2, 3):
+        got = _launch_run(sc, state, (25,), mode, precision)
+        for a, b in zip(got, ref):
+            assert same_bits(a, b)
+
+
+def test_launch_depth_rule():
+    """LBMX_LAUNCH_AUTO streams rows once the lattice gives k_stream 512
+    workgroups, and runs the tile kernels on smaller lattices; lattices with walls
+    take one step per launch."""
+    from lbc_amd import native
+    cases = [(2048, 2048, 0, 4), (256, 256, 0, 2), (1024, 1024, 0, 4), (2048, 2048, 1, 8),
+             (1024, 1024, 1, 3), (256, 256, 1, 3), (2048, 2048, 2, 4)]
+    for Nx, Ny, prec, depth in cases:
+        lat = native.Lattice(Nx, Ny, precision=prec, per_x=1, per_y=1)
+        lat.set_uniform(1.0 / 0.8)
+        lat.set_boundaries([("periodic_x", 0), ("periodic_y", 0)])
+        assert lat.launch_depth() == depth, (Nx, Ny, prec)
+        lat.set_launch(native.LAUNCH_SINGLE)
+        assert lat.launch_depth() == 1
+        lat.set_launch(native.LAUNCH_STREAM)
+        assert lat.launch_depth() == (8 if prec == 1 else 4)
+        lat.close()
+    lat = native.Lattice(64, 32, precision=0, per_x=1, per_y=0)
+    mask = np.zeros((32, 64), np.int32)
+    mask[0, :] = mask[-1, :] = 1
+    lat.set_geometry(mask, np.full((32, 64), 1.25), None)
+    lat.set_boundaries([("periodic_x", 0)])
+    assert lat.launch_depth() == 1
+    lat.close()
+
+
 @pytest.mark.parametrize("precision", [1, 2], ids=["f32", "f32wide"])
 @pytest.mark.parametrize("sc", GATHER_CASES, ids=[s.name for s in GATHER_CASES])
 def test_gather_form_fp32_equals_push(sc, precision):
