@@ -1227,10 +1227,10 @@ static void launch_pull3(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
 }
 
 // ---- row streaming (k_stream): D steps per launch ---------------------------------
-// Depth, prefetch and rows per workgroup were measured on the periodic Taylor-Green
-// lattice at 1024^2 .. 16384^2: FP64 4 steps (LDS 53.6 KB, three workgroups per
-// CU), FP32 storage 8 steps with FP32 collision (two workgroups of 8 wavefronts)
-// and 4 with FP64 collision.
+// Depth, prefetch, ring and rows per workgroup were measured on the periodic
+// Taylor-Green lattice at 1024^2 .. 16384^2: FP64 4 steps (three-row rings, LDS
+// 40.2 KB, four workgroups per CU), FP32 storage 8 steps with FP32 collision (two
+// workgroups of 8 wavefronts) and 4 with FP64 collision.
 constexpr int kStreamLW = 64;
 
 static int stream_depth(const lbmx_lattice *L)
@@ -1262,15 +1262,15 @@ static bool stream_fits(const lbmx_lattice *L, int depth)
     return wgs >= 512;
 }
 
-template <typename T, bool W, int D, int P>
+template <typename T, bool W, int D, int P, int R>
 static void launch_stream(lbmx_lattice *L, const T *S, T *Dst, StreamArgs &a)
 {
-    constexpr int TX = StreamShape<D, kStreamLW>::TX;
+    constexpr int TX = StreamShape<D, kStreamLW, R>::TX;
     a.strips = (L->Nx + TX - 1) / TX;
     a.seg_rows = stream_rows(L, D);
     a.segs = (L->Ny + a.seg_rows - 1) / a.seg_rows;
     const unsigned blocks = ((unsigned)a.strips * (unsigned)a.segs + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_stream<T, W, D, kStreamLW, P>), dim3(blocks), dim3(kStreamLW * D), 0,
+    hipLaunchKernelGGL((k_stream<T, W, D, kStreamLW, P, R>), dim3(blocks), dim3(kStreamLW * D), 0,
                        L->stream, S, Dst, a);
 }
 
@@ -1289,12 +1289,14 @@ static int pull_stream(lbmx_lattice *L)
     a.Fy_lag = s.Fy_lag;
     const T *S = (const T *)L->pop[L->pstar];
     T *D = (T *)L->pop[L->pstar ^ 1];
+    // FP64: three-row rings (40.2 KB of LDS, four workgroups per CU, 120 VGPRs) with
+    // three rows of loads in flight; the FP32 modes: four-row rings
     if constexpr (sizeof(T) == 8)
-        launch_stream<T, false, 4, 4>(L, S, D, a);
+        launch_stream<T, false, 4, 3, 3>(L, S, D, a);
     else if (g.wide)
-        launch_stream<T, true, 4, 4>(L, S, D, a);
+        launch_stream<T, true, 4, 4, 4>(L, S, D, a);
     else
-        launch_stream<T, false, 8, 6>(L, S, D, a);
+        launch_stream<T, false, 8, 6, 4>(L, S, D, a);
     HIPCHK(hipGetLastError());
     L->pstar ^= 1;
     s.Fx_lag = L->cfg.Fx;

@@ -309,45 +309,50 @@ __device__ __forceinline__ void gather_row(const T *__restrict__ S, const Geo &g
     }
 }
 
-template <int D, int LW>
+template <int D, int LW, int R>
 struct StreamShape {
     static constexpr int TX = LW - 2 * (D - 1);                  // output columns
     static constexpr int width(int l) { return LW - 2 * (l - 1); }
     static constexpr int offset(int l)                           // LDS values before level l
     {
         int o = 0;
-        for (int m = 1; m < l; m++) o += 4 * Q * width(m);
+        for (int m = 1; m < l; m++) o += R * Q * width(m);
         return o;
     }
     static constexpr int lds = offset(D);                        // levels 1..D-1
+    // ring slot of row r (r - rbase >= -2(D-1))
+    static __device__ __forceinline__ int slot(int r, int rbase)
+    {
+        return (int)((unsigned)(r - rbase + R * D) % (unsigned)R);
+    }
 };
 
-template <typename T, bool W, int D, int LW, int L>
+// Level L (2..D) of k_stream.  With a four-row ring (R = 4) a level writes the slot
+// of a row nobody reads in the same iteration: one barrier per iteration.  With a
+// three-row ring the new row replaces the oldest, which the next level reads in the
+// same iteration, so the write waits for a first barrier and a second one publishes
+// it.
+template <typename T, bool W, int D, int LW, int R, int L>
 __device__ __forceinline__ void stream_level(T *buf, const T *__restrict__ S, T *__restrict__ Dst,
                                              const StreamArgs &a, int lane, int x0, int y0, int y1,
                                              int iters)
 {
-    using Sh = StreamShape<D, LW>;
+    using Sh = StreamShape<D, LW, R>;
     constexpr int Win = Sh::width(L - 1), Wout = Sh::width(L);
     const Geo &g = a.g;
     const int rbase = y0 - (D - 1);
     const T *in = buf + Sh::offset(L - 1);
     for (int t = 0; t < iters; t++) {
         const int r = rbase + t - 2 * (L - 1);
-        const bool active = r >= y0 - (D - L) && r <= y1 - 1 + (D - L);
-        if (active && lane < Wout) {
-            T p[Q];
+        const bool active = r >= y0 - (D - L) && r <= y1 - 1 + (D - L) && lane < Wout;
+        T p[Q];
+        if (active) {
 #pragma unroll
-            for (int k = 0; k < Q; k++) {
-                const int s = (r - D2Q9::cy[k] - rbase + 4 * D) & 3;
-                p[k] = in[(s * Q + k) * Win + lane + 1 - D2Q9::cx[k]];
-            }
+            for (int k = 0; k < Q; k++)
+                p[k] = in[(Sh::slot(r - D2Q9::cy[k], rbase) * Q + k) * Win + lane + 1 -
+                          D2Q9::cx[k]];
             pull_collide<T, W>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
-            if constexpr (L < D) {
-                T *out = buf + Sh::offset(L) + ((r - rbase + 4 * D) & 3) * Q * Wout + lane;
-#pragma unroll
-                for (int k = 0; k < Q; k++) out[k * Wout] = p[k];
-            } else {
+            if constexpr (L == D) {
                 const int x = x0 + lane;
                 if (x < g.Nx) {
                     char *row = reinterpret_cast<char *>(Dst + (size_t)r * g.pitch);
@@ -359,27 +364,36 @@ __device__ __forceinline__ void stream_level(T *buf, const T *__restrict__ S, T 
                 }
             }
         }
+        if constexpr (R == 3) __syncthreads();
+        if constexpr (L < D) {
+            if (active) {
+                T *out = buf + Sh::offset(L) + Sh::slot(r, rbase) * Q * Wout + lane;
+#pragma unroll
+                for (int k = 0; k < Q; k++) out[k * Wout] = p[k];
+            }
+        }
         __syncthreads();
     }
 }
 
-template <typename T, bool W, int D, int LW, int... Ls>
+template <typename T, bool W, int D, int LW, int R, int... Ls>
 __device__ __forceinline__ void stream_levels(int level, T *buf, const T *__restrict__ S,
                                               T *__restrict__ Dst, const StreamArgs &a, int lane,
                                               int x0, int y0, int y1, int iters,
                                               std::integer_sequence<int, Ls...>)
 {
-    ((level == Ls + 2 ? stream_level<T, W, D, LW, Ls + 2>(buf, S, Dst, a, lane, x0, y0, y1, iters)
-                      : void()),
+    ((level == Ls + 2
+          ? stream_level<T, W, D, LW, R, Ls + 2>(buf, S, Dst, a, lane, x0, y0, y1, iters)
+          : void()),
      ...);
 }
 
-template <typename T, bool W, int D, int LW, int P>
+template <typename T, bool W, int D, int LW, int P, int R>
 __global__ void __launch_bounds__(LW * D) k_stream(const T *__restrict__ S, T *__restrict__ Dst,
                                                   StreamArgs a)
 {
-    using Sh = StreamShape<D, LW>;
-    static_assert(LW % 64 == 0 && Sh::TX > 0, "strip shape");
+    using Sh = StreamShape<D, LW, R>;
+    static_assert(LW % 64 == 0 && Sh::TX > 0 && (R == 3 || R == 4), "strip shape");
     __shared__ T buf[Sh::lds];
     const Geo &g = a.g;
     const int level = __builtin_amdgcn_readfirstlane(threadIdx.x / LW) + 1;
@@ -393,8 +407,8 @@ __global__ void __launch_bounds__(LW * D) k_stream(const T *__restrict__ S, T *_
     const int y0 = (item / a.strips) * a.seg_rows;
     const int y1 = min(y0 + a.seg_rows, g.Ny);
     const int x0 = (item % a.strips) * Sh::TX;
-    // every wavefront runs the same number of iterations (a multiple of P), one
-    // barrier each; iterations past a level's rows do nothing
+    // every wavefront runs the same number of iterations (a multiple of P) with the
+    // same barriers; iterations past a level's rows do nothing
     const int iters = ((y1 - y0) + 3 * (D - 1) + P - 1) / P * P;
     const int rbase = y0 - (D - 1);
 
@@ -414,9 +428,10 @@ __global__ void __launch_bounds__(LW * D) k_stream(const T *__restrict__ S, T *_
 #pragma unroll
             for (int j = 0; j < P; j++) {
                 const int t = t0 + j;
+                if (t < n1) pull_collide<T, W>(pre[j], g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
+                if constexpr (R == 3) __syncthreads();
                 if (t < n1) {
-                    pull_collide<T, W>(pre[j], g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
-                    T *out = buf + ((t + 4 * D) & 3) * Q * LW + lane;
+                    T *out = buf + Sh::slot(rbase + t, rbase) * Q * LW + lane;
 #pragma unroll
                     for (int k = 0; k < Q; k++) out[k * LW] = pre[j][k];
                 }
@@ -425,8 +440,8 @@ __global__ void __launch_bounds__(LW * D) k_stream(const T *__restrict__ S, T *_
             }
         }
     } else {
-        stream_levels<T, W, D, LW>(level, buf, S, Dst, a, lane, x0, y0, y1, iters,
-                                   std::make_integer_sequence<int, D - 1>{});
+        stream_levels<T, W, D, LW, R>(level, buf, S, Dst, a, lane, x0, y0, y1, iters,
+                                      std::make_integer_sequence<int, D - 1>{});
     }
 }
 
