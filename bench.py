@@ -25,7 +25,9 @@ Also reported in the same JSON line:
   cpu_baseline  the reference library (oracle/_ref, compiled from lbc's own
                 sources with its flags) timed on this host through main_lbm on a
                 bounded sample of the same workload
-  secondary     FP32-storage MLUPS at 16384^2 (FP32 collision and FP64 collision),
+  secondary     FP32-storage MLUPS at 16384^2 (FP32 collision and FP64 collision)
+                and both modes' velocity error against FP64 after 2,000 steps
+                at 4096^2,
                 a plain device copy's GB/s (the two-buffer ceiling), the roofline
                 sweep 1024^2 .. 16384^2 in FP64 and FP32, the 256^2 Taylor-Green
                 case against the analytic decay,
@@ -149,6 +151,29 @@ def tgv256_bench(steps=10000):
     return {"workload": f"taylor_green_periodic_256x256_fp64_{steps}_steps",
             "MLUPS": round(N * N * steps / wall / 1e6, 1), "seconds": round(wall, 3),
             "l2_velocity_error_vs_analytic": float(f"{err:.3e}")}
+
+
+def fp32_accuracy(N=4096, steps=2000):
+    """Relative L2 velocity error of both FP32 storage modes against FP64 on the
+    periodic Taylor-Green lattice N^2 after `steps` steps (the same multi-step
+    kernels as the speed lines)."""
+    from lbc_amd import native
+
+    def vel(prec):
+        lat = tgv_lattice(N, prec)
+        lat.step(steps)
+        _, u, v = lat.moments()
+        lat.close()
+        return u, v
+
+    u64, v64 = vel(native.F64)
+    den = float((u64 ** 2 + v64 ** 2).sum())
+    out = {"workload": f"taylor_green_periodic_{N}x{N}_{steps}_steps"}
+    for prec, key in ((native.F32_SHIFTED, "fp32_collision_vel_l2_vs_fp64"),
+                      (native.F32_WIDE, "fp64_collision_vel_l2_vs_fp64")):
+        u, v = vel(prec)
+        out[key] = float(f"{math.sqrt(float(((u - u64) ** 2 + (v - v64) ** 2).sum()) / den):.3e}")
+    return out
 
 
 def cylinder_lattice(precision):
@@ -610,6 +635,8 @@ def main():
             "steps_per_launch": splw,
             "vs_fp64": round(mw / mlups, 3)}
         log(f"FP32 storage, FP64 collision {N}^2: {mw:.0f} MLUPS ({mw / mlups:.2f}x FP64)")
+        secondary["fp32_accuracy"] = fp32_accuracy()
+        log(f"FP32 storage accuracy: {secondary['fp32_accuracy']}")
         copy = copy_ceiling()
         hbm = traffic / (launch_ms * 1e-3) / 1e9 if traffic else None
         secondary["copy_ceiling"] = {

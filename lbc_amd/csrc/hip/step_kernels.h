@@ -105,23 +105,36 @@ struct MomentsF {
 };
 
 // Explicit fmaf() everywhere a fused multiply-add is wanted (and no contraction
-// pragma), so every kernel that inlines these rounds identically.  1/rho is the
-// hardware reciprocal (within 1 ulp); the velocity it scales carries FP32
-// rounding of the same order anyway.
+// pragma), so every kernel that inlines these rounds identically.
+//
+// Nothing here rounds a quantity to FP32's grid around rho0 (spacing ~6e-8 near
+// 1).  Such a rounding repeats identically at a slowly changing node step after
+// step, so its error accumulates in the velocity instead of averaging out: with
+// m.dr = (rho0 + dr) - rho0 and u = j / (rho0 + dr) the velocity drifted from the
+// FP64 run by ~3e-8 of u per step on a Taylor-Green vortex.  So dr stays the raw
+// sum, and u = (j / rho0)(1 + q) with q = -dr / (rho0 + dr), whose rounding is
+// relative to dr, not to rho0.
 __device__ __forceinline__ MomentsF moments_f32(const float g[Q], float rho0, float Fx,
                                                 float Fy)
 {
     const float s56 = g[5] + g[6], s78 = g[7] + g[8];
     float dr = (((g[0] + g[1]) + (g[2] + g[3])) + g[4]) + (s56 + s78);
-    float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
-    float jy = ((g[2] - g[4]) + s56) - s78;
+    const float jx = __fmaf_rn(0.5f, Fx, (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]));
+    const float jy = __fmaf_rn(0.5f, Fy, ((g[2] - g[4]) + s56) - s78);
     float rho = rho0 + dr;
-    if (rho < 1e-16f) rho = 1e-16f;
-    const float inv = __builtin_amdgcn_rcpf(rho);
+    if (rho < 1e-16f) {
+        rho = 1e-16f;
+        dr = rho - rho0;
+    }
+    const float q = -dr * __builtin_amdgcn_rcpf(rho);
     MomentsF m;
-    m.dr = rho - rho0;
-    m.ux = __fmaf_rn(0.5f, Fx, jx) * inv;
-    m.uy = __fmaf_rn(0.5f, Fy, jy) * inv;
+    m.dr = dr;
+    m.ux = __fmaf_rn(jx, q, jx);
+    m.uy = __fmaf_rn(jy, q, jy);
+    if (rho0 != 1.0f) {
+        m.ux = m.ux / rho0;
+        m.uy = m.uy / rho0;
+    }
     return m;
 }
 
@@ -131,16 +144,20 @@ __device__ __forceinline__ float collide_f32_one(float gk, const MomentsF &m, fl
                                                  float Fy, float uF3, float gf)
 {
     constexpr int cx = D2Q9::cx[K], cy = D2Q9::cy[K];
-    const float w = (float)D2Q9::w[K];
-    float geq;
-    float cu = 0.0f;
+    // the FP32 weights all sit 7.5e-9 above 4/9, 1/9, 1/36: w*X alone would scale
+    // the equilibrium's momentum by that much every step, so the product is formed
+    // with the weight's remainder and rounded once
+    constexpr float w = (float)D2Q9::w[K];
+    constexpr float wlo = (float)(D2Q9::w[K] - (double)w);
+    float cu = 0.0f, X;
     if constexpr (K == 0) {
-        geq = w * __fmaf_rn(rho, -usq15, m.dr);
+        X = __fmaf_rn(rho, -usq15, m.dr);
     } else {
         cu = cdot<cx, cy>(m.ux, m.uy);
         const float t = __fmaf_rn(cu, __fmaf_rn(4.5f, cu, 3.0f), -usq15);
-        geq = w * __fmaf_rn(rho, t, m.dr);
+        X = __fmaf_rn(rho, t, m.dr);
     }
+    const float geq = __fmaf_rn(w, X, wlo * X);
     float r = __fmaf_rn(-om, gk - geq, gk);
     if (force) {
         float s;
