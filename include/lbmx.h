@@ -140,7 +140,7 @@ LBMX_API int lbmx_step(lbmx_lattice *lat, int nsteps);
  *                       it at least 512 workgroups, else tiles (the default)
  *   LBMX_LAUNCH_SINGLE  one step per launch (also the default when the
  *                       environment sets LBMX_NO_TEMPORAL)
- *   LBMX_LAUNCH_TILES   two (FP64) or three (FP32 storage) steps per launch on
+ *   LBMX_LAUNCH_TILES   three steps per launch (two with LBMX_TEMPORAL_DEPTH=2) on
  *                       64 x TY tiles with their rings in LDS
  *   LBMX_LAUNCH_STREAM  4 (FP64, FP32 storage with FP64 collision) or 8 (FP32)
  *                       steps per launch, each workgroup streaming down a column

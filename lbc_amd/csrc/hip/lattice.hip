@@ -1419,7 +1419,7 @@ extern "C" int lbmx_launch_depth(const lbmx_lattice *L)
     static int depth_env = -1;
     if (depth_env < 0)
         depth_env = getenv("LBMX_TEMPORAL_DEPTH") ? atoi(getenv("LBMX_TEMPORAL_DEPTH")) : 0;
-    return depth_env ? depth_env : (L->esize == 8 ? 2 : 3);
+    return depth_env ? depth_env : 3;
 }
 
 extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
@@ -1440,12 +1440,13 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
         if (rc) return rc;
     }
     int i = 0;
-    // tile depth: 2 steps per launch in FP64, 3 with FP32 storage (measured fastest
-    // at 16384^2); LBMX_TEMPORAL_DEPTH=2 or 3 overrides
+    // tile depth: 3 steps per launch (the tiles serve the lattices too small for
+    // row streaming, where three beat two in FP64 as in FP32: 256^2 3.69 vs 3.97
+    // us per step, 512^2 4.66 vs 5.05); LBMX_TEMPORAL_DEPTH=2 or 3 overrides
     static int depth_env = -1;
     if (depth_env < 0)
         depth_env = getenv("LBMX_TEMPORAL_DEPTH") ? atoi(getenv("LBMX_TEMPORAL_DEPTH")) : 0;
-    const int depth = depth_env ? depth_env : (f64 ? 2 : 3);
+    const int depth = depth_env ? depth_env : 3;
     const int mode = L->launch_mode;
     if (L->pull && mode != LBMX_LAUNCH_SINGLE) {
         // multi-step launches while more steps remain than they take: the run ends

@@ -158,7 +158,7 @@ def _launch_run(sc, state, chunks, mode, precision=0):
 @pytest.mark.parametrize("sc", LAUNCH_CASES, ids=[s.name for s in LAUNCH_CASES])
 @pytest.mark.parametrize("mode", [1, 2, 3], ids=["single", "tiles", "stream"])
 def test_launch_modes_bit_identical(sc, mode):
-    """One step per launch, tiles (2 steps per launch) and row streaming (4 steps
+    """One step per launch, tiles (3 steps per launch) and row streaming (4 steps
     per launch, forced on these small lattices), over runs of 19 and 6 + 11 steps:
     every buffer and moment equals the oracle bit for bit."""
     state = sc.initial_state()
@@ -190,7 +190,7 @@ def test_launch_depth_rule():
     workgroups, and runs the tile kernels on smaller lattices; lattices with walls
     take one step per launch."""
     from lbc_amd import native
-    cases = [(2048, 2048, 0, 4), (256, 256, 0, 2), (1024, 1024, 0, 4), (2048, 2048, 1, 8),
+    cases = [(2048, 2048, 0, 4), (256, 256, 0, 3), (1024, 1024, 0, 4), (2048, 2048, 1, 8),
              (1024, 1024, 1, 3), (256, 256, 1, 3), (2048, 2048, 2, 4)]
     for Nx, Ny, prec, depth in cases:
         lat = native.Lattice(Nx, Ny, precision=prec, per_x=1, per_y=1)
