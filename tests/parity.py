@@ -7,8 +7,8 @@ import numpy as np
 import oracle as O
 from cases import Scenario
 
-# North-star FP64 tolerance (relative, in rho and u).  The FP64 HIP path is
-# expected to be bit-identical; the tolerance is what the tests enforce.
+# Required FP64 agreement with the reference (relative, in rho and u).  The FP64
+# HIP path is expected to be bit-identical; the tolerance is what the tests enforce.
 FP64_RTOL = 1e-12
 
 
