@@ -154,6 +154,12 @@ LBMX_API int lbmx_set_launch(lbmx_lattice *lat, int mode);
 /* Steps per launch of a long run of this lattice under its mode (1 when the
  * lattice does not qualify). */
 LBMX_API int lbmx_launch_depth(const lbmx_lattice *lat);
+/* The same rule for a qualifying lattice of this size and precision, on the host
+ * (no device needed): *steps_per_launch, and for row streaming the output
+ * columns (*columns) and rows (*rows) of one workgroup's strip segment; both 0
+ * for the tile and one-step forms. */
+LBMX_API int lbmx_launch_plan(int Nx, int Ny, int precision, int mode, int *steps_per_launch,
+                              int *columns, int *rows);
 
 /* Time-domain impedance boundary (TDIBC) driven on the device every step, the
  * scheme of cases/nit/nit.c:13-47 and :275-303: before each step the wall

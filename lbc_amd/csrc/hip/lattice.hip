@@ -1233,33 +1233,75 @@ static void launch_pull3(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
 // workgroups of 8 wavefronts) and 4 with FP64 collision.
 constexpr int kStreamLW = 64;
 
-static int stream_depth(const lbmx_lattice *L)
+static int stream_depth_of(int precision)
 {
-    return (L->esize == 4 && L->cfg.precision != LBMX_F32_WIDE) ? 8 : 4;
+    return precision == LBMX_F32_SHIFTED ? 8 : 4;
 }
+static int stream_depth(const lbmx_lattice *L) { return stream_depth_of(L->cfg.precision); }
 
 // Rows per workgroup: about 4096 workgroups over the lattice, as a power of two
 // between 32 and 128 at depth 4, and between 128 (the depth-8 pipeline takes 21
 // iterations to fill) and 512 at depth 8.
-static int stream_rows(const lbmx_lattice *L, int depth)
+static int stream_rows_of(int Nx, int Ny, int depth)
 {
     const int tx = kStreamLW - 2 * (depth - 1);
-    const double strips = (double)((L->Nx + tx - 1) / tx);
-    const double want = strips * L->Ny / 4096.0;
+    const double strips = (double)((Nx + tx - 1) / tx);
+    const double want = strips * Ny / 4096.0;
     const int lo = depth <= 4 ? 32 : 128, hi = depth <= 4 ? 128 : 512;
     int rows = lo;
     while (rows < hi && want >= rows * 1.41421356) rows *= 2;
-    return std::min(rows, L->Ny);
+    return std::min(rows, Ny);
+}
+static int stream_rows(const lbmx_lattice *L, int depth)
+{
+    return stream_rows_of(L->Nx, L->Ny, depth);
 }
 
 // k_stream pays off once it has at least 512 workgroups (two per CU); smaller
 // lattices run the tile kernels.
-static bool stream_fits(const lbmx_lattice *L, int depth)
+static bool stream_fits_of(int Nx, int Ny, int depth)
 {
     const int tx = kStreamLW - 2 * (depth - 1);
-    const int rows = stream_rows(L, depth);
-    const long long wgs = (long long)((L->Nx + tx - 1) / tx) * ((L->Ny + rows - 1) / rows);
+    const int rows = stream_rows_of(Nx, Ny, depth);
+    const long long wgs = (long long)((Nx + tx - 1) / tx) * ((Ny + rows - 1) / rows);
     return wgs >= 512;
+}
+static bool stream_fits(const lbmx_lattice *L, int depth)
+{
+    return stream_fits_of(L->Nx, L->Ny, depth);
+}
+
+static int tile_depth()
+{
+    static int depth_env = -1;
+    if (depth_env < 0)
+        depth_env = getenv("LBMX_TEMPORAL_DEPTH") ? atoi(getenv("LBMX_TEMPORAL_DEPTH")) : 0;
+    return depth_env == 2 ? 2 : 3;
+}
+
+extern "C" int lbmx_launch_plan(int Nx, int Ny, int precision, int mode, int *steps_per_launch,
+                                int *columns, int *rows)
+{
+    if (!steps_per_launch || !columns || !rows) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (Nx < 3 || Ny < 3) return lbmx_fail(LBMX_ERR_ARG, "lattice %dx%d too small", Nx, Ny);
+    if (precision < LBMX_F64 || precision > LBMX_F32_WIDE)
+        return lbmx_fail(LBMX_ERR_ARG, "unknown precision %d", precision);
+    if (mode < LBMX_LAUNCH_AUTO || mode > LBMX_LAUNCH_STREAM)
+        return lbmx_fail(LBMX_ERR_ARG, "unknown launch mode %d", mode);
+    *columns = *rows = 0;
+    if (mode == LBMX_LAUNCH_SINGLE) {
+        *steps_per_launch = 1;
+        return LBMX_OK;
+    }
+    const int sd = stream_depth_of(precision);
+    if (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && stream_fits_of(Nx, Ny, sd))) {
+        *steps_per_launch = sd;
+        *columns = kStreamLW - 2 * (sd - 1);
+        *rows = stream_rows_of(Nx, Ny, sd);
+        return LBMX_OK;
+    }
+    *steps_per_launch = tile_depth();
+    return LBMX_OK;
 }
 
 template <typename T, bool W, int D, int P, int R>
@@ -1411,15 +1453,10 @@ extern "C" int lbmx_set_launch(lbmx_lattice *L, int mode)
 extern "C" int lbmx_launch_depth(const lbmx_lattice *L)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
-    if (!pull_eligible(L) || L->launch_mode == LBMX_LAUNCH_SINGLE) return 1;
-    const int sd = stream_depth(L);
-    if (L->launch_mode == LBMX_LAUNCH_STREAM ||
-        (L->launch_mode == LBMX_LAUNCH_AUTO && stream_fits(L, sd)))
-        return sd;
-    static int depth_env = -1;
-    if (depth_env < 0)
-        depth_env = getenv("LBMX_TEMPORAL_DEPTH") ? atoi(getenv("LBMX_TEMPORAL_DEPTH")) : 0;
-    return depth_env ? depth_env : 3;
+    if (!pull_eligible(L)) return 1;
+    int d = 1, c = 0, r = 0;
+    const int rc = lbmx_launch_plan(L->Nx, L->Ny, L->cfg.precision, L->launch_mode, &d, &c, &r);
+    return rc ? rc : d;
 }
 
 extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
@@ -1443,10 +1480,7 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
     // tile depth: 3 steps per launch (the tiles serve the lattices too small for
     // row streaming, where three beat two in FP64 as in FP32: 256^2 3.69 vs 3.97
     // us per step, 512^2 4.66 vs 5.05); LBMX_TEMPORAL_DEPTH=2 or 3 overrides
-    static int depth_env = -1;
-    if (depth_env < 0)
-        depth_env = getenv("LBMX_TEMPORAL_DEPTH") ? atoi(getenv("LBMX_TEMPORAL_DEPTH")) : 0;
-    const int depth = depth_env ? depth_env : 3;
+    const int depth = tile_depth();
     const int mode = L->launch_mode;
     if (L->pull && mode != LBMX_LAUNCH_SINGLE) {
         // multi-step launches while more steps remain than they take: the run ends

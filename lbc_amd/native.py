@@ -90,6 +90,7 @@ def _bind():
     L.lbmx_time_steps.argtypes = [vp, i, _dp]
     L.lbmx_set_launch.argtypes = [vp, i]
     L.lbmx_launch_depth.argtypes = [vp]
+    L.lbmx_launch_plan.argtypes = [i, i, i, i, _ip, _ip, _ip]
     L.lbmx_set_tdibc.argtypes = [vp, C.POINTER(Tdibc)]
     L.lbmx_tdibc_history.argtypes = [vp, _dp, i]
     L.lbmx_set_probes.argtypes = [vp, _ip, _ip, i, i]
@@ -160,6 +161,16 @@ def copy_rate(gib=8.0):
     out = C.c_double()
     check(_bind().lbmx_copy_rate(float(gib), C.byref(out)))
     return out.value
+
+
+def launch_plan(Nx, Ny, precision=F64, mode=LAUNCH_AUTO):
+    """(steps per launch, strip columns, rows per workgroup) of a long multi-step
+    run of a periodic all-fluid lattice (lbmx_launch_plan; host only)."""
+    L = _bind()
+    d, c, r = C.c_int(), C.c_int(), C.c_int()
+    check(L.lbmx_launch_plan(int(Nx), int(Ny), int(precision), int(mode), C.byref(d),
+                             C.byref(c), C.byref(r)))
+    return d.value, c.value, r.value
 
 
 def tail_plan(Nx, Ny, per_x, per_y, mask, bcs):

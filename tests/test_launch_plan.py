@@ -1,0 +1,49 @@
+"""The launch rule of multi-step runs on periodic all-fluid lattices
+(lbmx_launch_plan, host only): which kernel family a size and precision get,
+and the strip and segment shape of row streaming."""
+import pytest
+
+from lbc_amd import native
+
+
+@pytest.mark.parametrize("Nx,Ny,prec,depth", [
+    (2048, 2048, native.F64, 4), (256, 256, native.F64, 3), (1024, 1024, native.F64, 4),
+    (512, 512, native.F64, 3), (16384, 16384, native.F64, 4),
+    (2048, 2048, native.F32_SHIFTED, 8), (1024, 1024, native.F32_SHIFTED, 3),
+    (16384, 16384, native.F32_SHIFTED, 8), (2048, 2048, native.F32_WIDE, 4),
+    (8192, 256, native.F64, 4), (64, 100000, native.F64, 4)])
+def test_auto_rule(Nx, Ny, prec, depth):
+    d, cols, rows = native.launch_plan(Nx, Ny, prec)
+    assert d == depth
+    if d in (4, 8):
+        # strip of 64 - 2(D - 1) columns; rows a power of two in the depth's range
+        assert cols == 64 - 2 * (d - 1)
+        lo, hi = (32, 128) if d == 4 else (128, 512)
+        assert rows == min(Ny, rows) and (rows & (rows - 1)) == 0 and lo <= rows <= hi
+        strips = -(-Nx // cols)
+        assert strips * -(-Ny // rows) >= 512      # streamed only with >= 512 workgroups
+    else:
+        assert cols == 0 and rows == 0
+
+
+def test_headline_shape():
+    # 16384^2 FP64: 283 strips of 58 columns x 128 segments of 128 rows
+    assert native.launch_plan(16384, 16384, native.F64) == (4, 58, 128)
+    assert native.launch_plan(16384, 16384, native.F32_SHIFTED) == (8, 50, 512)
+
+
+def test_forced_modes():
+    for prec, sd in ((native.F64, 4), (native.F32_SHIFTED, 8), (native.F32_WIDE, 4)):
+        assert native.launch_plan(300, 200, prec, native.LAUNCH_SINGLE) == (1, 0, 0)
+        assert native.launch_plan(16384, 16384, prec, native.LAUNCH_TILES) == (3, 0, 0)
+        d, cols, rows = native.launch_plan(3, 150, prec, native.LAUNCH_STREAM)
+        assert d == sd and cols == 64 - 2 * (sd - 1) and rows <= 150
+
+
+def test_bad_arguments():
+    with pytest.raises(native.LbmxError):
+        native.launch_plan(2, 100)
+    with pytest.raises(native.LbmxError):
+        native.launch_plan(100, 100, 7)
+    with pytest.raises(native.LbmxError):
+        native.launch_plan(100, 100, native.F64, 9)
