@@ -416,3 +416,31 @@ def test_body_force_record(sc):
     np.testing.assert_allclose([Lb, Db], ref_body[-1], rtol=1e-11, atol=1e-12)
     if (sc.phi < -1e29).any():                         # the walls do carry force
         assert np.abs(ref_body - ref_total).max() > 1e-6
+
+
+@pytest.mark.parametrize("precision", [1, 2], ids=["f32", "f32wide"])
+@pytest.mark.parametrize("mode", [1, 3], ids=["single", "stream"])
+def test_fp32_storage_with_other_shift_density(precision, mode):
+    """FP32 storage shifted by rho0 = 1.3 on a state of density ~1.3 (the FP32
+    velocity then divides by rho0 as well): one step per launch and row streaming
+    within the FP32 bars of the FP64 oracle."""
+    from lbc_amd import native
+    sc = LAUNCH_CASES[3]
+    f, fn, rho, u, v = sc.initial_state()
+    s = 1.3
+    state = (f * s, fn * s, rho * s, u, v)
+    ref, _ = run_oracle(sc, state, 17)
+    lat = native.Lattice(sc.Nx, sc.Ny, precision=precision, per_x=1, per_y=1, Fx=sc.Fx,
+                         Fy=sc.Fy, rho0=s)
+    lat.set_launch(mode)
+    lat.set_geometry(sc.mask, sc.omega, sc.phi)
+    lat.set_boundaries(sc.bc_names)
+    lat.upload(*state)
+    lat.step(17)
+    r, uu, vv = lat.moments()
+    lat.close()
+    fluid = sc.mask == 0
+    du = np.sqrt(np.sum((uu - ref.u)[fluid] ** 2 + (vv - ref.v)[fluid] ** 2))
+    nu = np.sqrt(np.sum(ref.u[fluid] ** 2 + ref.v[fluid] ** 2))
+    assert du / nu <= FP32_VEL_L2[precision], du / nu
+    assert rel_err(r, ref.rho, fluid) <= 1e-6
