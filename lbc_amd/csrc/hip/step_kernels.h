@@ -573,7 +573,7 @@ __device__ __forceinline__ void link_value(const T *__restrict__ S, const LinkAr
 }
 
 // Links [blk * kLinkThreads, +kLinkThreads) and their force partial sum, which goes
-// to partials[blk]; the partials are summed in index order by bc_force_total.
+// to partials[blk]; bc_force_total sums the partials in a fixed order.
 template <typename T>
 __device__ __forceinline__ void links_block(const T *__restrict__ S, T *__restrict__ D,
                                             const LinkArgs &a, int blk)
@@ -1127,16 +1127,35 @@ __device__ __forceinline__ void bc_apply(const T *S, T *D, const BcArgs &a, cons
 // the record row {t, L, D, L_body, D_body}
 constexpr int kRecordCols = 5;
 
+// Called by every thread of one workgroup of NT threads (a power of two): thread t
+// sums the partials t, t + NT, ... in that order, then a fixed pairwise tree in LDS
+// adds the NT sums, so the result does not depend on timing while all threads'
+// loads are in flight together.
+template <int NT>
 __device__ __forceinline__ void bc_force_total(const BcArgs &a)
 {
+    static_assert((NT & (NT - 1)) == 0, "power of two");
+    __shared__ double red[kPartial][NT];
+    const int t = threadIdx.x;
     double s[kPartial] = {0.0, 0.0, 0.0, 0.0};
-    if (a.npartials > 0) {
-        for (int p = 0; p < a.npartials; p++)
+    for (int p = t; p < a.npartials; p += NT)
 #pragma unroll
-            for (int c = 0; c < kPartial; c++) s[c] += a.partials[kPartial * p + c];
+        for (int c = 0; c < kPartial; c++) s[c] += a.partials[kPartial * p + c];
+#pragma unroll
+    for (int c = 0; c < kPartial; c++) red[c][t] = s[c];
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int c = 0; c < kPartial; c++) red[c][t] += red[c][t + w];
+        __syncthreads();
+    }
+    if (t != 0) return;
+#pragma unroll
+    for (int c = 0; c < kPartial; c++) s[c] = red[c][0];
+    if (a.npartials > 0)
 #pragma unroll
         for (int c = 0; c < kPartial; c++) a.force_out[c] = s[c];
-    }
     if (a.record && a.row < a.capacity) {
         double *r = a.record + (size_t)kRecordCols * a.row;
         r[0] = (double)a.row * a.dt;
@@ -1156,7 +1175,7 @@ __global__ void __launch_bounds__(kBcThreads) k_bc(const T *__restrict__ S, T *_
         bc_apply(S, D, a, a.bcs[b], Span{(int)threadIdx.x, (int)blockDim.x});
         __syncthreads();
     }
-    if (threadIdx.x == 0) bc_force_total(a);
+    bc_force_total<kBcThreads>(a);
 }
 
 // ---- k_edges: stale slots of non-periodic domain edges ----------------------------
@@ -1252,7 +1271,7 @@ __global__ void __launch_bounds__(kTailThreads) k_tail(T *__restrict__ S, T *__r
             for (int e = sp.first; e < a.nrow + a.ncol; e += sp.stride) edge_node(S, ea, e);
         }
     }
-    if (a.with_force && blk == 0 && threadIdx.x == 0) bc_force_total(a.bc);
+    if (a.with_force && blk == 0) bc_force_total<kTailThreads>(a.bc);   // whole block 0
 }
 
 // ---- k_materialize: the reference's post-collision buffer, on demand ----------------
