@@ -444,3 +444,13 @@ def test_fp32_storage_with_other_shift_density(precision, mode):
     nu = np.sqrt(np.sum(ref.u[fluid] ** 2 + ref.v[fluid] ** 2))
     assert du / nu <= FP32_VEL_L2[precision], du / nu
     assert rel_err(r, ref.rho, fluid) <= 1e-6
+
+
+def test_force_total_is_deterministic():
+    """The force total is a fixed-order reduction over the link partials: two runs
+    of the same scenario give the same forces bit for bit, every step."""
+    sc = FORCE_CASES[0]
+    state = sc.initial_state()
+    a = run_gpu(sc, state, NSTEPS)["forces"]
+    b = run_gpu(sc, state, NSTEPS)["forces"]
+    assert np.array_equal(a, b)
