@@ -273,12 +273,13 @@ __global__ void __launch_bounds__(kPull2Threads) k_pull3(const T *__restrict__ S
 // the ring level l still needs); level 1 gathers from S in global memory, levels
 // 2..D gather from level l-1's rows in LDS, and level D stores f*_{s+D} to D.
 // Level l works two rows behind level l-1, so everything an iteration reads was
-// written in an earlier one: one barrier per iteration, and each level keeps its
-// last four rows in an LDS ring.  Level 1 keeps P rows of loads in flight in
-// registers.  Compared with the tiles of k_pull2/k_pull3, no row is computed
-// twice (only the 2(D-1) ring columns of a strip are), and the loads of one
-// wavefront overlap the arithmetic of the others.  Same arithmetic on the same
-// values as D k_pull launches: bit-identical.
+// written in an earlier one.  Each level keeps its last R rows in an LDS ring:
+// R = 4 needs one barrier per iteration, R = 3 two (the new row replaces the
+// oldest once the first barrier says it has been read).  Level 1 keeps P rows of
+// loads in flight in registers.  Compared with the tiles of k_pull2/k_pull3, no
+// row is computed twice (only the 2(D-1) ring columns of a strip are), and the
+// loads of one wavefront overlap the arithmetic of the others.  Same arithmetic
+// on the same values as D k_pull launches: bit-identical.
 struct StreamArgs {
     Geo g;
     int force;
