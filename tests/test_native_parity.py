@@ -136,6 +136,12 @@ LAUNCH_CASES = GATHER_CASES + [
              Fx=2e-6, Fy=1e-6),
     Scenario("tgv_thin", 3, 150, [("periodic_x", 0), ("periodic_y", 0)], 35),
     Scenario("tgv_flat", 130, 3, [("periodic_x", 0), ("periodic_y", 0)], 36),
+    # strips that tile the width exactly (58 FP64 / 50 FP32 columns), and a strip
+    # wider than the lattice
+    Scenario("tgv_exact_strips", 116, 41, [("periodic_x", 0), ("periodic_y", 0)], 37),
+    Scenario("tgv_exact_strips32", 100, 9, [("periodic_x", 0), ("periodic_y", 0)], 38,
+             Fx=-1e-6),
+    Scenario("tgv_narrow", 17, 64, [("periodic_x", 0), ("periodic_y", 0)], 39),
 ]
 
 
@@ -454,3 +460,17 @@ def test_force_total_is_deterministic():
     a = run_gpu(sc, state, NSTEPS)["forces"]
     b = run_gpu(sc, state, NSTEPS)["forces"]
     assert np.array_equal(a, b)
+
+
+
+def test_set_launch_rejects_unknown_mode():
+    from lbc_amd import native
+    lat = native.Lattice(64, 32, per_x=1, per_y=1)
+    try:
+        with pytest.raises(native.LbmxError):
+            lat.set_launch(7)
+        with pytest.raises(native.LbmxError):
+            lat.set_launch(-1)
+        lat.set_launch(native.LAUNCH_STREAM)   # a valid mode still works after
+    finally:
+        lat.close()
