@@ -545,6 +545,10 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--size", type=int, default=16384)
+    ap.add_argument("--precision", choices=("f64", "f32", "f32wide"), default="f64",
+                    help="storage/arithmetic of the headline lattice (default f64, the "
+                         "reference's arithmetic; f32 = FP32 storage and collision, f32wide = "
+                         "FP32 storage, FP64 collision) -- for profiling one kernel")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=2048)
@@ -585,7 +589,11 @@ def main():
     cells = N * N
     log(f"device: {native.device_name()}")
 
-    lat = tgv_lattice(N, native.F64)
+    prec, bpc, ptag, pname = {
+        "f64": (native.F64, 144, "fp64", "FP64"),
+        "f32": (native.F32_SHIFTED, 72, "fp32", "FP32-storage"),
+        "f32wide": (native.F32_WIDE, 72, "fp32wide", "FP32-storage FP64-collision")}[args.precision]
+    lat = tgv_lattice(N, prec)
     wall, dev_ms = time_lattice(lat, args.steps, args.warmup)
     # the dominant kernel and its launch time: a run of D*m + 1 steps is m D-step
     # launches and one one-step launch (k_pull), so the difference of two such runs
@@ -599,14 +607,15 @@ def main():
         launch_ms = (t_a - t_b) / (m_a - m_b)
     else:
         launch_ms = dev_ms / args.steps
-    kernel = FP64_KERNELS.get(steps_per_launch, f"{steps_per_launch} steps per launch")
+    kernel = (FP64_KERNELS.get(steps_per_launch, f"{steps_per_launch} steps per launch")
+              if prec == native.F64 else f"{pname} gather form, {steps_per_launch} steps per launch")
     lat.close()
     mlups = cells * args.steps / wall / 1e6
-    achieved = 144.0 * cells * steps_per_launch / (launch_ms * 1e-3) / 1e9
-    traffic = load_traffic(f"tgv{N}_fp64" + (f"_{steps_per_launch}step" if temporal else ""))
-    log(f"FP64 {N}^2: {wall / args.steps * 1e3:.3f} ms/step wall, "
+    achieved = float(bpc) * cells * steps_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = load_traffic(f"tgv{N}_{ptag}" + (f"_{steps_per_launch}step" if temporal else ""))
+    log(f"{pname} {N}^2: {wall / args.steps * 1e3:.3f} ms/step wall, "
         f"{dev_ms / args.steps:.3f} ms/step events, {mlups:.0f} MLUPS; {kernel}: "
-        f"{launch_ms:.3f} ms per launch, {achieved:.0f} GB/s algorithmic (144 B per "
+        f"{launch_ms:.3f} ms per launch, {achieved:.0f} GB/s algorithmic ({bpc} B per "
         f"cell-update)")
 
     secondary = {}
@@ -668,14 +677,14 @@ def main():
     secondary["headline_kernel"] = {
         "kernel": kernel, "steps_per_launch": steps_per_launch,
         "launch_ms": round(launch_ms, 4),
-        "algorithmic_bytes_per_launch": 144 * cells * steps_per_launch,
+        "algorithmic_bytes_per_launch": bpc * cells * steps_per_launch,
         "hbm_bytes_per_launch": traffic,
         "hbm_GB_per_s": round(traffic / (launch_ms * 1e-3) / 1e9, 1) if traffic else None,
         "note": f"{steps_per_launch} steps per HBM pass move less than the per-step "
                 "algorithmic bytes, so the per-step roofline fraction can exceed 1"
                 if temporal else ""}
     out = {
-        "metric": "MLUPS (FP64 fused D2Q9 BGK step) + % HBM roofline",
+        "metric": f"MLUPS ({pname} fused D2Q9 BGK step) + % HBM roofline",
         "value": round(mlups, 1),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -685,9 +694,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"f64": "f64", "f32": "f32", "f32wide": "f32 storage, f64 arithmetic"}[args.precision],
         "data": "synthetic (Taylor-Green vortex generated on the device)",
-        "config": {"workload": f"taylor_green_periodic_{N}x{N}_fp64", "Nx": N, "Ny": N,
+        "config": {"workload": f"taylor_green_periodic_{N}x{N}_{ptag}", "Nx": N, "Ny": N,
                    "tau": 0.8, "u0": 0.01, "lattice": "D2Q9 BGK", "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
