@@ -37,6 +37,7 @@ Also reported in the same JSON line:
                 requests), each against sequential single-lattice FP64 solves
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--size N]
+                    [--precision f64|f32|f32wide]
 """
 from __future__ import annotations
 

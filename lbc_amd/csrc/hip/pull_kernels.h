@@ -421,7 +421,14 @@ __global__ void __launch_bounds__(LW * D) k_stream(const T *__restrict__ S, T *_
         const unsigned bx = (unsigned)x * (unsigned)sizeof(T);
         const unsigned bxm = (unsigned)(x == 0 ? g.Nx - 1 : x - 1) * (unsigned)sizeof(T);
         const unsigned bxp = (unsigned)(x == g.Nx - 1 ? 0 : x + 1) * (unsigned)sizeof(T);
-        auto row = [&](int t) { return ((rbase + min(t, n1 - 1)) % g.Ny + g.Ny) % g.Ny; };
+        // periodic row of level-1 iteration t; the rows lie within D - 1 of the
+        // lattice, so the wrap is a few conditional additions (no integer division)
+        auto row = [&](int t) {
+            int r = rbase + min(t, n1 - 1);
+            while (r < 0) r += g.Ny;
+            while (r >= g.Ny) r -= g.Ny;
+            return r;
+        };
         T pre[P][Q];
 #pragma unroll
         for (int j = 0; j < P; j++) gather_row(S, g, row(j), bxm, bx, bxp, pre[j]);
