@@ -155,9 +155,9 @@ LBMX_API int lbmx_set_launch(lbmx_lattice *lat, int mode);
  * lattice does not qualify). */
 LBMX_API int lbmx_launch_depth(const lbmx_lattice *lat);
 /* The same rule for a qualifying lattice of this size and precision, on the host
- * (no device needed): *steps_per_launch, and for row streaming the output
- * columns (*columns) and rows (*rows) of one workgroup's strip segment; both 0
- * for the tile and one-step forms. */
+ * (no device needed): *steps_per_launch, and the output columns (*columns) and
+ * rows (*rows) of one workgroup's strip segment (row streaming) or tile (the
+ * tile form); both 0 for one step per launch. */
 LBMX_API int lbmx_launch_plan(int Nx, int Ny, int precision, int mode, int *steps_per_launch,
                               int *columns, int *rows);
 

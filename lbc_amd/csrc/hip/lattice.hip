@@ -1279,6 +1279,22 @@ static int tile_depth()
     return depth_env == 2 ? 2 : 3;
 }
 
+// Rows per tile of the multi-step tile kernels (kPull2TX columns each).  Tile
+// heights measured at 16384^2 (ms per step): FP32 8 (1.70; 4: 1.69, 16: 1.70),
+// FP32 storage + FP64 collision 16 (2.38; 8: 2.57), FP64 8.  The three-step tiles
+// serve the lattices too small for row streaming, where more, shorter tiles pay
+// off: the tallest of 8, 4, 2 rows that still gives 1024 workgroups (256^2: 2
+// rows, FP64 19.6 vs 17.8 GLUPS with 8, FP32 26.4 vs 21.6; 512^2: 4 rows, FP32 77
+// vs 72; 768^2 and up: 8 rows).
+static int tile_rows_of(int Nx, int Ny, int precision, int depth)
+{
+    if (precision == LBMX_F32_WIDE) return 16;
+    if (depth == 2) return 8;
+    const long long tiles_x = (Nx + kPull2TX - 1) / kPull2TX;
+    auto blocks = [&](int ty) { return tiles_x * ((Ny + ty - 1) / ty); };
+    return blocks(8) >= 1024 ? 8 : (blocks(4) >= 1024 ? 4 : 2);
+}
+
 extern "C" int lbmx_launch_plan(int Nx, int Ny, int precision, int mode, int *steps_per_launch,
                                 int *columns, int *rows)
 {
@@ -1301,6 +1317,10 @@ extern "C" int lbmx_launch_plan(int Nx, int Ny, int precision, int mode, int *st
         return LBMX_OK;
     }
     *steps_per_launch = tile_depth();
+    if (mode == LBMX_LAUNCH_TILES || mode == LBMX_LAUNCH_AUTO) {
+        *columns = kPull2TX;
+        *rows = tile_rows_of(Nx, Ny, precision, *steps_per_launch);
+    }
     return LBMX_OK;
 }
 
@@ -1366,15 +1386,22 @@ static int pull_step3(lbmx_lattice *L)
     a.tiles_x = (L->Nx + kPull2TX - 1) / kPull2TX;
     const T *S = (const T *)L->pop[L->pstar];
     T *D = (T *)L->pop[L->pstar ^ 1];
-    // tile heights measured at 16384^2 (ms per step): FP32 8 (1.70; 4: 1.69, 16:
-    // 1.70), FP32 storage + FP64 collision 16 (2.38; 8: 2.57), FP64 8 (3.81; the
-    // two-step kernel, 3.72, is faster there and is the FP64 default)
-    if constexpr (sizeof(T) == 8)
-        launch_pull3<T, false, 8>(L, S, D, a);
-    else if (g.wide)
-        launch_pull3<T, true, 16>(L, S, D, a);
-    else
-        launch_pull3<T, false, 8>(L, S, D, a);
+    const int ty = tile_rows_of(L->Nx, L->Ny, L->cfg.precision, 3);
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+        if (g.wide) {
+            launch_pull3<T, true, 16>(L, S, D, a);
+            done = true;
+        }
+    }
+    if (!done) {
+        if (ty == 8)
+            launch_pull3<T, false, 8>(L, S, D, a);
+        else if (ty == 4)
+            launch_pull3<T, false, 4>(L, S, D, a);
+        else
+            launch_pull3<T, false, 2>(L, S, D, a);
+    }
     HIPCHK(hipGetLastError());
     L->pstar ^= 1;
     s.Fx_lag = L->cfg.Fx;
@@ -1404,7 +1431,7 @@ static int pull_step2(lbmx_lattice *L)
     // tile heights (LDS of a workgroup: 9 x 66 x (TY + 2) values, under 64 KB) as
     // measured fastest at 16384^2 among 2-24: FP64 8 (3.72 ms per step; 4: 3.82,
     // 10: 4.23), FP32 8 (1.91; 16: 1.94, 24: 2.25), FP32 storage + FP64
-    // collision 16 (2.57; 8: 2.72)
+    // collision 16 (2.57; 8: 2.72); tile_rows_of(..., 2) reports them
     if constexpr (sizeof(T) == 8)
         launch_pull2<T, false, 8>(L, S, D, a);
     else if (g.wide)

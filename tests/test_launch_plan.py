@@ -1,6 +1,7 @@
 """The launch rule of multi-step runs on periodic all-fluid lattices
 (lbmx_launch_plan, host only): which kernel family a size and precision get,
-and the strip and segment shape of row streaming."""
+the strip and segment shape of row streaming and the tile shape of the
+three-step tiles."""
 import pytest
 
 from lbc_amd import native
@@ -23,7 +24,22 @@ def test_auto_rule(Nx, Ny, prec, depth):
         strips = -(-Nx // cols)
         assert strips * -(-Ny // rows) >= 512      # streamed only with >= 512 workgroups
     else:
-        assert cols == 0 and rows == 0
+        # tiles of 64 columns: 16 rows with FP64 collision on FP32 storage, else
+        # the tallest of 8, 4, 2 rows that gives >= 1024 tiles (2 if none does)
+        assert cols == 64
+        if prec == native.F32_WIDE:
+            assert rows == 16
+        else:
+            tiles = lambda ty: -(-Nx // 64) * -(-Ny // ty)
+            assert rows in (8, 4, 2) and (tiles(rows) >= 1024 or rows == 2)
+            assert rows == 8 or tiles(2 * rows) < 1024
+
+
+@pytest.mark.parametrize("N,rows", [(256, 2), (300, 2), (512, 4), (600, 4), (768, 8), (1000, 8)])
+def test_tile_rows(N, rows):
+    for prec in (native.F64, native.F32_SHIFTED):
+        assert native.launch_plan(N, N, prec, native.LAUNCH_TILES) == (3, 64, rows)
+    assert native.launch_plan(N, N, native.F32_WIDE, native.LAUNCH_TILES) == (3, 64, 16)
 
 
 def test_headline_shape():
@@ -35,7 +51,8 @@ def test_headline_shape():
 def test_forced_modes():
     for prec, sd in ((native.F64, 4), (native.F32_SHIFTED, 8), (native.F32_WIDE, 4)):
         assert native.launch_plan(300, 200, prec, native.LAUNCH_SINGLE) == (1, 0, 0)
-        assert native.launch_plan(16384, 16384, prec, native.LAUNCH_TILES) == (3, 0, 0)
+        assert native.launch_plan(16384, 16384, prec, native.LAUNCH_TILES) == \
+            (3, 64, 16 if prec == native.F32_WIDE else 8)
         d, cols, rows = native.launch_plan(3, 150, prec, native.LAUNCH_STREAM)
         assert d == sd and cols == 64 - 2 * (sd - 1) and rows <= 150
 

@@ -142,6 +142,10 @@ LAUNCH_CASES = GATHER_CASES + [
     Scenario("tgv_exact_strips32", 100, 9, [("periodic_x", 0), ("periodic_y", 0)], 38,
              Fx=-1e-6),
     Scenario("tgv_narrow", 17, 64, [("periodic_x", 0), ("periodic_y", 0)], 39),
+    # the three-step tiles at their other heights: 4 rows (1104 tiles; the cases
+    # above get 2), and 8 rows (1161 tiles), the last tile row partial in both
+    Scenario("tgv_tiles4", 200, 1101, [("periodic_x", 0), ("periodic_y", 0)], 40, Fy=1e-6),
+    Scenario("tgv_tiles8", 520, 1030, [("periodic_x", 0), ("periodic_y", 0)], 41),
 ]
 
 
