@@ -1257,18 +1257,24 @@ static int stream_rows(const lbmx_lattice *L, int depth)
     return stream_rows_of(L->Nx, L->Ny, depth);
 }
 
-// k_stream pays off once it has at least 512 workgroups (two per CU); smaller
-// lattices run the tile kernels.
-static bool stream_fits_of(int Nx, int Ny, int depth)
+// k_stream beats the three-step tiles once it has enough workgroups: 320 (FP64
+// and FP32), 400 with FP32 storage and FP64 collision, whose 16-row tiles stay
+// strong longer.  Measured, GLUPS tiles vs stream (workgroups): FP64 704^2 54.8
+// vs 46.0 (286), 768^2 51.4 vs 54.5 (336), 896^2 54.5 vs 73.8 (448); FP32
+// 1280^2 121.9 vs 105.6 (260), 1408^2 123.8 vs 126.7 (319), 1536^2 134.3 vs
+// 149.7 (372); FP32 + FP64 collision 768^2 75.7 vs 63.3 (336), 896^2 72.3 vs
+// 85.8 (448).  Smaller lattices run the tile kernels.
+static bool stream_fits_of(int Nx, int Ny, int precision)
 {
+    const int depth = stream_depth_of(precision);
     const int tx = kStreamLW - 2 * (depth - 1);
     const int rows = stream_rows_of(Nx, Ny, depth);
     const long long wgs = (long long)((Nx + tx - 1) / tx) * ((Ny + rows - 1) / rows);
-    return wgs >= 512;
+    return wgs >= (precision == LBMX_F32_WIDE ? 400 : 320);
 }
-static bool stream_fits(const lbmx_lattice *L, int depth)
+static bool stream_fits(const lbmx_lattice *L)
 {
-    return stream_fits_of(L->Nx, L->Ny, depth);
+    return stream_fits_of(L->Nx, L->Ny, L->cfg.precision);
 }
 
 static int tile_depth()
@@ -1310,7 +1316,7 @@ extern "C" int lbmx_launch_plan(int Nx, int Ny, int precision, int mode, int *st
         return LBMX_OK;
     }
     const int sd = stream_depth_of(precision);
-    if (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && stream_fits_of(Nx, Ny, sd))) {
+    if (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && stream_fits_of(Nx, Ny, precision))) {
         *steps_per_launch = sd;
         *columns = kStreamLW - 2 * (sd - 1);
         *rows = stream_rows_of(Nx, Ny, sd);
@@ -1513,7 +1519,7 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
         // multi-step launches while more steps remain than they take: the run ends
         // with pull_step
         const int sd = stream_depth(L);
-        if (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && stream_fits(L, sd)))
+        if (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && stream_fits(L)))
             for (; nsteps - i >= sd + 1; i += sd) {
                 int rc = f64 ? pull_stream<double>(L) : pull_stream<float>(L);
                 if (rc) return rc;

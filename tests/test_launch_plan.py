@@ -12,7 +12,11 @@ from lbc_amd import native
     (512, 512, native.F64, 3), (16384, 16384, native.F64, 4),
     (2048, 2048, native.F32_SHIFTED, 8), (1024, 1024, native.F32_SHIFTED, 3),
     (16384, 16384, native.F32_SHIFTED, 8), (2048, 2048, native.F32_WIDE, 4),
-    (8192, 256, native.F64, 4), (64, 100000, native.F64, 4)])
+    (8192, 256, native.F64, 4), (64, 100000, native.F64, 4),
+    # around the streaming threshold (320 workgroups; 400 for F32_WIDE)
+    (704, 704, native.F64, 3), (768, 768, native.F64, 4), (768, 768, native.F32_WIDE, 3),
+    (896, 896, native.F32_WIDE, 4), (1408, 1408, native.F32_SHIFTED, 3),
+    (1536, 1536, native.F32_SHIFTED, 8)])
 def test_auto_rule(Nx, Ny, prec, depth):
     d, cols, rows = native.launch_plan(Nx, Ny, prec)
     assert d == depth
@@ -22,7 +26,8 @@ def test_auto_rule(Nx, Ny, prec, depth):
         lo, hi = (32, 128) if d == 4 else (128, 512)
         assert rows == min(Ny, rows) and (rows & (rows - 1)) == 0 and lo <= rows <= hi
         strips = -(-Nx // cols)
-        assert strips * -(-Ny // rows) >= 512      # streamed only with >= 512 workgroups
+        # streamed only with >= 320 workgroups (400 with FP64 collision on FP32 storage)
+        assert strips * -(-Ny // rows) >= (400 if prec == native.F32_WIDE else 320)
     else:
         # tiles of 64 columns: 16 rows with FP64 collision on FP32 storage, else
         # the tallest of 8, 4, 2 rows that gives >= 1024 tiles (2 if none does)

@@ -196,12 +196,13 @@ def test_launch_modes_fp32_identical(sc, precision):
 
 
 def test_launch_depth_rule():
-    """LBMX_LAUNCH_AUTO streams rows once the lattice gives k_stream 512
-    workgroups, and runs the tile kernels on smaller lattices; lattices with walls
-    take one step per launch."""
+    """LBMX_LAUNCH_AUTO streams rows once the lattice gives k_stream 320
+    workgroups (400 with FP64 collision on FP32 storage), and runs the tile
+    kernels on smaller lattices; lattices with walls take one step per launch."""
     from lbc_amd import native
     cases = [(2048, 2048, 0, 4), (256, 256, 0, 3), (1024, 1024, 0, 4), (2048, 2048, 1, 8),
-             (1024, 1024, 1, 3), (256, 256, 1, 3), (2048, 2048, 2, 4)]
+             (1024, 1024, 1, 3), (256, 256, 1, 3), (2048, 2048, 2, 4), (768, 768, 0, 4),
+             (704, 704, 0, 3), (768, 768, 2, 3), (896, 896, 2, 4), (1536, 1536, 1, 8)]
     for Nx, Ny, prec, depth in cases:
         lat = native.Lattice(Nx, Ny, precision=prec, per_x=1, per_y=1)
         lat.set_uniform(1.0 / 0.8)
