@@ -1233,12 +1233,6 @@ static void launch_pull3(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
 // workgroups of 8 wavefronts) and 4 with FP64 collision.
 constexpr int kStreamLW = 64;
 
-static int stream_depth_of(int precision)
-{
-    return precision == LBMX_F32_SHIFTED ? 8 : 4;
-}
-static int stream_depth(const lbmx_lattice *L) { return stream_depth_of(L->cfg.precision); }
-
 // Rows per workgroup: about 4096 workgroups over the lattice, as a power of two
 // between 32 and 128 at depth 4, and between 128 (the depth-8 pipeline takes 21
 // iterations to fill) and 512 at depth 8.
@@ -1257,20 +1251,45 @@ static int stream_rows(const lbmx_lattice *L, int depth)
     return stream_rows_of(L->Nx, L->Ny, depth);
 }
 
-// k_stream beats the three-step tiles once it has enough workgroups: 320 (FP64
-// and FP32), 400 with FP32 storage and FP64 collision, whose 16-row tiles stay
-// strong longer.  Measured, GLUPS tiles vs stream (workgroups): FP64 704^2 54.8
-// vs 46.0 (286), 768^2 51.4 vs 54.5 (336), 896^2 54.5 vs 73.8 (448); FP32
-// 1280^2 121.9 vs 105.6 (260), 1408^2 123.8 vs 126.7 (319), 1536^2 134.3 vs
-// 149.7 (372); FP32 + FP64 collision 768^2 75.7 vs 63.3 (336), 896^2 72.3 vs
-// 85.8 (448).  Smaller lattices run the tile kernels.
-static bool stream_fits_of(int Nx, int Ny, int precision)
+// k_stream beats the three-step tiles once it has enough workgroups: 320 in FP64
+// and at eight steps, 400 for the four-step FP32-storage kernels.  Measured,
+// GLUPS tiles vs stream (workgroups): FP64 704^2 54.8 vs 46.0 (286), 768^2 51.4
+// vs 54.5 (336), 896^2 54.5 vs 73.8 (448); FP32 eight steps 1280^2 121.9 vs
+// 105.6 (260), 1408^2 123.8 vs 126.7 (319), 1536^2 134.3 vs 149.7 (372); FP32
+// four steps 768^2 100.0 vs 96.2 (336), 896^2 101.8 vs 128.8 (448), 1024^2 114.5
+// vs 130.5; FP32 + FP64 collision 768^2 75.7 vs 63.3 (336), 896^2 72.3 vs 85.8
+// (448).  Smaller lattices run the tile kernels.
+static long long stream_wgs_of(int Nx, int Ny, int depth)
 {
-    const int depth = stream_depth_of(precision);
     const int tx = kStreamLW - 2 * (depth - 1);
     const int rows = stream_rows_of(Nx, Ny, depth);
-    const long long wgs = (long long)((Nx + tx - 1) / tx) * ((Ny + rows - 1) / rows);
-    return wgs >= (precision == LBMX_F32_WIDE ? 400 : 320);
+    return (long long)((Nx + tx - 1) / tx) * ((Ny + rows - 1) / rows);
+}
+static long long stream_threshold(int precision, int depth)
+{
+    return precision == LBMX_F64 || depth == 8 ? 320 : 400;
+}
+
+// Steps per streaming launch: 4; 8 with FP32 storage and FP32 collision from 6 M
+// cells on, and whenever the caller forces streaming.  The four-step kernel is the
+// faster one while the two FP32 population buffers mostly fit the 256 MB Infinity
+// Cache, GLUPS four vs eight steps: 1408^2 163 vs 127, 1536^2 175 vs 151, 2048^2
+// 192 vs 173; 3072^2 168 vs 205, 4096^2 153 vs 206, 8192^2 154 vs 218.
+static int stream_depth_of(int Nx, int Ny, int precision, int mode)
+{
+    if (precision != LBMX_F32_SHIFTED) return 4;
+    if (mode == LBMX_LAUNCH_STREAM || (long long)Nx * Ny >= 6000000) return 8;
+    return 4;
+}
+static int stream_depth(const lbmx_lattice *L)
+{
+    return stream_depth_of(L->Nx, L->Ny, L->cfg.precision, L->launch_mode);
+}
+
+static bool stream_fits_of(int Nx, int Ny, int precision)
+{
+    const int depth = stream_depth_of(Nx, Ny, precision, LBMX_LAUNCH_AUTO);
+    return stream_wgs_of(Nx, Ny, depth) >= stream_threshold(precision, depth);
 }
 static bool stream_fits(const lbmx_lattice *L)
 {
@@ -1315,7 +1334,7 @@ extern "C" int lbmx_launch_plan(int Nx, int Ny, int precision, int mode, int *st
         *steps_per_launch = 1;
         return LBMX_OK;
     }
-    const int sd = stream_depth_of(precision);
+    const int sd = stream_depth_of(Nx, Ny, precision, mode);
     if (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && stream_fits_of(Nx, Ny, precision))) {
         *steps_per_launch = sd;
         *columns = kStreamLW - 2 * (sd - 1);
@@ -1363,8 +1382,10 @@ static int pull_stream(lbmx_lattice *L)
         launch_stream<T, false, 4, 3, 3>(L, S, D, a);
     else if (g.wide)
         launch_stream<T, true, 4, 4, 4>(L, S, D, a);
-    else
+    else if (stream_depth(L) == 8)
         launch_stream<T, false, 8, 6, 4>(L, S, D, a);
+    else
+        launch_stream<T, false, 4, 4, 4>(L, S, D, a);
     HIPCHK(hipGetLastError());
     L->pstar ^= 1;
     s.Fx_lag = L->cfg.Fx;

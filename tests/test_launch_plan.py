@@ -10,13 +10,16 @@ from lbc_amd import native
 @pytest.mark.parametrize("Nx,Ny,prec,depth", [
     (2048, 2048, native.F64, 4), (256, 256, native.F64, 3), (1024, 1024, native.F64, 4),
     (512, 512, native.F64, 3), (16384, 16384, native.F64, 4),
-    (2048, 2048, native.F32_SHIFTED, 8), (1024, 1024, native.F32_SHIFTED, 3),
+    (2048, 2048, native.F32_SHIFTED, 4), (1024, 1024, native.F32_SHIFTED, 4),
+    (3072, 3072, native.F32_SHIFTED, 8), (768, 768, native.F32_SHIFTED, 3),
     (16384, 16384, native.F32_SHIFTED, 8), (2048, 2048, native.F32_WIDE, 4),
     (8192, 256, native.F64, 4), (64, 100000, native.F64, 4),
     # around the streaming threshold (320 workgroups; 400 for F32_WIDE)
     (704, 704, native.F64, 3), (768, 768, native.F64, 4), (768, 768, native.F32_WIDE, 3),
-    (896, 896, native.F32_WIDE, 4), (1408, 1408, native.F32_SHIFTED, 3),
-    (1536, 1536, native.F32_SHIFTED, 8)])
+    (896, 896, native.F32_WIDE, 4), (1408, 1408, native.F32_SHIFTED, 4),
+    (1536, 1536, native.F32_SHIFTED, 4), (896, 896, native.F32_SHIFTED, 4),
+    (16384, 256, native.F32_SHIFTED, 4), (2448, 2448, native.F32_SHIFTED, 4),
+    (2450, 2450, native.F32_SHIFTED, 8)])
 def test_auto_rule(Nx, Ny, prec, depth):
     d, cols, rows = native.launch_plan(Nx, Ny, prec)
     assert d == depth
@@ -26,8 +29,11 @@ def test_auto_rule(Nx, Ny, prec, depth):
         lo, hi = (32, 128) if d == 4 else (128, 512)
         assert rows == min(Ny, rows) and (rows & (rows - 1)) == 0 and lo <= rows <= hi
         strips = -(-Nx // cols)
-        # streamed only with >= 320 workgroups (400 with FP64 collision on FP32 storage)
-        assert strips * -(-Ny // rows) >= (400 if prec == native.F32_WIDE else 320)
+        # streamed only with >= 320 workgroups in FP64 and at eight steps, 400 for
+        # the four-step FP32-storage kernels; FP32 eight steps from 6 M cells
+        assert strips * -(-Ny // rows) >= (320 if prec == native.F64 or d == 8 else 400)
+        if prec == native.F32_SHIFTED:
+            assert (d == 8) == (Nx * Ny >= 6_000_000)
     else:
         # tiles of 64 columns: 16 rows with FP64 collision on FP32 storage, else
         # the tallest of 8, 4, 2 rows that gives >= 1024 tiles (2 if none does)
@@ -51,6 +57,8 @@ def test_headline_shape():
     # 16384^2 FP64: 283 strips of 58 columns x 128 segments of 128 rows
     assert native.launch_plan(16384, 16384, native.F64) == (4, 58, 128)
     assert native.launch_plan(16384, 16384, native.F32_SHIFTED) == (8, 50, 512)
+    # forced streaming keeps eight FP32 steps at any size
+    assert native.launch_plan(1024, 1024, native.F32_SHIFTED, native.LAUNCH_STREAM)[0] == 8
 
 
 def test_forced_modes():

@@ -195,14 +195,31 @@ def test_launch_modes_fp32_identical(sc, precision):
             assert same_bits(a, b)
 
 
+def test_fp32_four_step_stream_identical():
+    """FP32 storage and collision at four steps per launch (the default on
+    mid-size lattices, 1000 x 1030 here: 18 strips x 33 segments, the last of
+    each partial) rounds exactly as one step per launch, with a force."""
+    from lbc_amd import native
+    sc = Scenario("tgv_f32_stream4", 1000, 1030, [("periodic_x", 0), ("periodic_y", 0)], 42,
+                  Fx=1e-6, Fy=-2e-6)
+    assert native.launch_plan(sc.Nx, sc.Ny, native.F32_SHIFTED) == (4, 58, 32)
+    state = sc.initial_state()
+    ref = _launch_run(sc, state, (14,), native.LAUNCH_SINGLE, native.F32_SHIFTED)
+    got = _launch_run(sc, state, (14,), native.LAUNCH_AUTO, native.F32_SHIFTED)
+    for a, b in zip(got, ref):
+        assert same_bits(a, b)
+
+
 def test_launch_depth_rule():
     """LBMX_LAUNCH_AUTO streams rows once the lattice gives k_stream 320
-    workgroups (400 with FP64 collision on FP32 storage), and runs the tile
-    kernels on smaller lattices; lattices with walls take one step per launch."""
+    workgroups (400 for the four-step FP32-storage kernels; FP32 eight steps from
+    6 M cells), and runs the tile kernels on smaller lattices; lattices with walls
+    take one step per launch."""
     from lbc_amd import native
-    cases = [(2048, 2048, 0, 4), (256, 256, 0, 3), (1024, 1024, 0, 4), (2048, 2048, 1, 8),
-             (1024, 1024, 1, 3), (256, 256, 1, 3), (2048, 2048, 2, 4), (768, 768, 0, 4),
-             (704, 704, 0, 3), (768, 768, 2, 3), (896, 896, 2, 4), (1536, 1536, 1, 8)]
+    cases = [(2048, 2048, 0, 4), (256, 256, 0, 3), (1024, 1024, 0, 4), (2048, 2048, 1, 4),
+             (1024, 1024, 1, 4), (256, 256, 1, 3), (2048, 2048, 2, 4), (768, 768, 0, 4),
+             (704, 704, 0, 3), (768, 768, 2, 3), (896, 896, 2, 4), (768, 768, 1, 3),
+             (3072, 3072, 1, 8)]
     for Nx, Ny, prec, depth in cases:
         lat = native.Lattice(Nx, Ny, precision=prec, per_x=1, per_y=1)
         lat.set_uniform(1.0 / 0.8)
