@@ -1236,7 +1236,7 @@ constexpr int kStreamLW = 64;
 // Rows per workgroup: about 4096 workgroups over the lattice, as a power of two
 // between 32 and 128 at depth 4, and between 128 (the depth-8 pipeline takes 21
 // iterations to fill) and 512 at depth 8.
-static int stream_rows_of(int Nx, int Ny, int depth)
+static int stream_rows_base(int Nx, int Ny, int depth)
 {
     const int tx = kStreamLW - 2 * (depth - 1);
     const double strips = (double)((Nx + tx - 1) / tx);
@@ -1245,10 +1245,6 @@ static int stream_rows_of(int Nx, int Ny, int depth)
     int rows = lo;
     while (rows < hi && want >= rows * 1.41421356) rows *= 2;
     return std::min(rows, Ny);
-}
-static int stream_rows(const lbmx_lattice *L, int depth)
-{
-    return stream_rows_of(L->Nx, L->Ny, depth);
 }
 
 // k_stream beats the three-step tiles once it has enough workgroups: 320 in FP64
@@ -1259,11 +1255,29 @@ static int stream_rows(const lbmx_lattice *L, int depth)
 // four steps 768^2 100.0 vs 96.2 (336), 896^2 101.8 vs 128.8 (448), 1024^2 114.5
 // vs 130.5; FP32 + FP64 collision 768^2 75.7 vs 63.3 (336), 896^2 72.3 vs 85.8
 // (448).  Smaller lattices run the tile kernels.
+// Workgroups at the base row count (the launch thresholds below were measured
+// with it).
 static long long stream_wgs_of(int Nx, int Ny, int depth)
 {
     const int tx = kStreamLW - 2 * (depth - 1);
-    const int rows = stream_rows_of(Nx, Ny, depth);
+    const int rows = stream_rows_base(Nx, Ny, depth);
     return (long long)((Nx + tx - 1) / tx) * ((Ny + rows - 1) / rows);
+}
+
+// Lattices that give fewer than 768 workgroups of 32 rows at depth 4 run 16-row
+// segments instead, which fill the CUs better despite the longer pipeline fill
+// per row; GLUPS 32 vs 16 rows: FP64 768^2 54.4 vs 66.5, 1024^2 75.4 vs 77.1,
+// 1280^2 (920 workgroups) 89.5 vs 85.0; FP32 1024^2 130.5 vs 140.9, 1280^2 167.2
+// vs 155.5; FP32 + FP64 collision 1024^2 87.1 vs 92.8, 1280^2 111.0 vs 97.5.
+static int stream_rows_of(int Nx, int Ny, int depth)
+{
+    const int rows = stream_rows_base(Nx, Ny, depth);
+    if (depth == 4 && rows == 32 && stream_wgs_of(Nx, Ny, depth) < 768) return 16;
+    return rows;
+}
+static int stream_rows(const lbmx_lattice *L, int depth)
+{
+    return stream_rows_of(L->Nx, L->Ny, depth);
 }
 static long long stream_threshold(int precision, int depth)
 {

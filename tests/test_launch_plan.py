@@ -26,12 +26,20 @@ def test_auto_rule(Nx, Ny, prec, depth):
     if d in (4, 8):
         # strip of 64 - 2(D - 1) columns; rows a power of two in the depth's range
         assert cols == 64 - 2 * (d - 1)
-        lo, hi = (32, 128) if d == 4 else (128, 512)
-        assert rows == min(Ny, rows) and (rows & (rows - 1)) == 0 and lo <= rows <= hi
         strips = -(-Nx // cols)
-        # streamed only with >= 320 workgroups in FP64 and at eight steps, 400 for
-        # the four-step FP32-storage kernels; FP32 eight steps from 6 M cells
-        assert strips * -(-Ny // rows) >= (320 if prec == native.F64 or d == 8 else 400)
+        lo, hi = (32, 128) if d == 4 else (128, 512)
+        if d == 4 and rows == 16:
+            # 16-row segments only where 32-row ones give fewer than 768 workgroups
+            assert strips * -(-Ny // 32) < 768
+            base = 32
+        else:
+            assert rows == min(Ny, rows) and (rows & (rows - 1)) == 0 and lo <= rows <= hi
+            assert not (d == 4 and rows == 32 and strips * -(-Ny // 32) < 768)
+            base = rows
+        # streamed only with >= 320 workgroups (at the base row count) in FP64 and
+        # at eight steps, 400 for the four-step FP32-storage kernels; FP32 eight
+        # steps from 6 M cells
+        assert strips * -(-Ny // base) >= (320 if prec == native.F64 or d == 8 else 400)
         if prec == native.F32_SHIFTED:
             assert (d == 8) == (Nx * Ny >= 6_000_000)
     else:
@@ -51,6 +59,14 @@ def test_tile_rows(N, rows):
     for prec in (native.F64, native.F32_SHIFTED):
         assert native.launch_plan(N, N, prec, native.LAUNCH_TILES) == (3, 64, rows)
     assert native.launch_plan(N, N, native.F32_WIDE, native.LAUNCH_TILES) == (3, 64, 16)
+
+
+@pytest.mark.parametrize("N,prec,rows", [
+    (768, native.F64, 16), (1024, native.F64, 16), (1280, native.F64, 32),
+    (1024, native.F32_SHIFTED, 16), (1280, native.F32_SHIFTED, 32),
+    (1024, native.F32_WIDE, 16), (1280, native.F32_WIDE, 32)])
+def test_small_stream_rows(N, prec, rows):
+    assert native.launch_plan(N, N, prec) == (4, 58, rows)
 
 
 def test_headline_shape():

@@ -197,12 +197,12 @@ def test_launch_modes_fp32_identical(sc, precision):
 
 def test_fp32_four_step_stream_identical():
     """FP32 storage and collision at four steps per launch (the default on
-    mid-size lattices, 1000 x 1030 here: 18 strips x 33 segments, the last of
-    each partial) rounds exactly as one step per launch, with a force."""
+    mid-size lattices, 1000 x 1030 here: 18 strips x 65 segments of 16 rows, the
+    last of each partial) rounds exactly as one step per launch, with a force."""
     from lbc_amd import native
     sc = Scenario("tgv_f32_stream4", 1000, 1030, [("periodic_x", 0), ("periodic_y", 0)], 42,
                   Fx=1e-6, Fy=-2e-6)
-    assert native.launch_plan(sc.Nx, sc.Ny, native.F32_SHIFTED) == (4, 58, 32)
+    assert native.launch_plan(sc.Nx, sc.Ny, native.F32_SHIFTED) == (4, 58, 16)
     state = sc.initial_state()
     ref = _launch_run(sc, state, (14,), native.LAUNCH_SINGLE, native.F32_SHIFTED)
     got = _launch_run(sc, state, (14,), native.LAUNCH_AUTO, native.F32_SHIFTED)
