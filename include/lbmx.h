@@ -137,8 +137,8 @@ LBMX_API int lbmx_step(lbmx_lattice *lat, int nsteps);
  * gives bit-identical results).  Such runs keep post-collision populations and
  * advance several steps per launch:
  *   LBMX_LAUNCH_AUTO    row streaming (LBMX_LAUNCH_STREAM) when the lattice gives
- *                       it at least 320 workgroups (400 for LBMX_F32_WIDE), else
- *                       tiles (the default)
+ *                       it enough workgroups (from about 600^2; 896^2 for
+ *                       LBMX_F32_WIDE), else tiles (the default)
  *   LBMX_LAUNCH_SINGLE  one step per launch (also the default when the
  *                       environment sets LBMX_NO_TEMPORAL)
  *   LBMX_LAUNCH_TILES   three steps per launch (two with LBMX_TEMPORAL_DEPTH=2) on

@@ -1247,14 +1247,6 @@ static int stream_rows_base(int Nx, int Ny, int depth)
     return std::min(rows, Ny);
 }
 
-// k_stream beats the three-step tiles once it has enough workgroups: 320 in FP64
-// and at eight steps, 400 for the four-step FP32-storage kernels.  Measured,
-// GLUPS tiles vs stream (workgroups): FP64 704^2 54.8 vs 46.0 (286), 768^2 51.4
-// vs 54.5 (336), 896^2 54.5 vs 73.8 (448); FP32 eight steps 1280^2 121.9 vs
-// 105.6 (260), 1408^2 123.8 vs 126.7 (319), 1536^2 134.3 vs 149.7 (372); FP32
-// four steps 768^2 100.0 vs 96.2 (336), 896^2 101.8 vs 128.8 (448), 1024^2 114.5
-// vs 130.5; FP32 + FP64 collision 768^2 75.7 vs 63.3 (336), 896^2 72.3 vs 85.8
-// (448).  Smaller lattices run the tile kernels.
 // Workgroups at the base row count (the launch thresholds below were measured
 // with it).
 static long long stream_wgs_of(int Nx, int Ny, int depth)
@@ -1281,7 +1273,7 @@ static int stream_rows(const lbmx_lattice *L, int depth)
 }
 static long long stream_threshold(int precision, int depth)
 {
-    return precision == LBMX_F64 || depth == 8 ? 320 : 400;
+    return precision == LBMX_F32_WIDE ? 400 : (depth == 8 ? 320 : 200);
 }
 
 // Steps per streaming launch: 4; 8 with FP32 storage and FP32 collision from 6 M
@@ -1300,6 +1292,15 @@ static int stream_depth(const lbmx_lattice *L)
     return stream_depth_of(L->Nx, L->Ny, L->cfg.precision, L->launch_mode);
 }
 
+// k_stream beats the three-step tiles once it has enough workgroups (counted at
+// the base row count): 200 at four steps (FP64, FP32), 320 at eight, 400 with
+// FP32 storage and FP64 collision, whose 16-row tiles stay strong longer.
+// Measured, GLUPS tiles vs stream (base workgroups): FP64 512^2 56.3 vs 38.8
+// (144), 576^2 47.4 vs 48.6 (180), 640^2 49.9 vs 59.9 (240), 704^2 54.5 vs 58.6
+// (286); FP32 four steps 640^2 81.5 vs 100.8 (240), 768^2 100.0 vs 114.8 (336);
+// FP32 eight steps 1280^2 121.9 vs 105.6 (260), 1536^2 134.3 vs 149.7 (372);
+// FP32 + FP64 collision 704^2 76.0 vs 66.7 (286), 832^2 87.2 vs 78.9 (390),
+// 896^2 72.3 vs 85.8 (448).  Smaller lattices run the tile kernels.
 static bool stream_fits_of(int Nx, int Ny, int precision)
 {
     const int depth = stream_depth_of(Nx, Ny, precision, LBMX_LAUNCH_AUTO);

@@ -11,11 +11,13 @@ from lbc_amd import native
     (2048, 2048, native.F64, 4), (256, 256, native.F64, 3), (1024, 1024, native.F64, 4),
     (512, 512, native.F64, 3), (16384, 16384, native.F64, 4),
     (2048, 2048, native.F32_SHIFTED, 4), (1024, 1024, native.F32_SHIFTED, 4),
-    (3072, 3072, native.F32_SHIFTED, 8), (768, 768, native.F32_SHIFTED, 3),
+    (3072, 3072, native.F32_SHIFTED, 8), (768, 768, native.F32_SHIFTED, 4),
+    (576, 576, native.F32_SHIFTED, 3), (640, 640, native.F32_SHIFTED, 4),
     (16384, 16384, native.F32_SHIFTED, 8), (2048, 2048, native.F32_WIDE, 4),
     (8192, 256, native.F64, 4), (64, 100000, native.F64, 4),
     # around the streaming threshold (320 workgroups; 400 for F32_WIDE)
-    (704, 704, native.F64, 3), (768, 768, native.F64, 4), (768, 768, native.F32_WIDE, 3),
+    (576, 576, native.F64, 3), (640, 640, native.F64, 4), (704, 704, native.F64, 4),
+    (768, 768, native.F64, 4), (768, 768, native.F32_WIDE, 3),
     (896, 896, native.F32_WIDE, 4), (1408, 1408, native.F32_SHIFTED, 4),
     (1536, 1536, native.F32_SHIFTED, 4), (896, 896, native.F32_SHIFTED, 4),
     (16384, 256, native.F32_SHIFTED, 4), (2448, 2448, native.F32_SHIFTED, 4),
@@ -36,10 +38,11 @@ def test_auto_rule(Nx, Ny, prec, depth):
             assert rows == min(Ny, rows) and (rows & (rows - 1)) == 0 and lo <= rows <= hi
             assert not (d == 4 and rows == 32 and strips * -(-Ny // 32) < 768)
             base = rows
-        # streamed only with >= 320 workgroups (at the base row count) in FP64 and
-        # at eight steps, 400 for the four-step FP32-storage kernels; FP32 eight
+        # streamed only with >= 200 workgroups (at the base row count) at four
+        # steps, 320 at eight, 400 with FP64 collision on FP32 storage; FP32 eight
         # steps from 6 M cells
-        assert strips * -(-Ny // base) >= (320 if prec == native.F64 or d == 8 else 400)
+        assert strips * -(-Ny // base) >= (400 if prec == native.F32_WIDE else 320 if d == 8
+                                           else 200)
         if prec == native.F32_SHIFTED:
             assert (d == 8) == (Nx * Ny >= 6_000_000)
     else:

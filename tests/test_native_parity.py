@@ -211,14 +211,15 @@ def test_fp32_four_step_stream_identical():
 
 
 def test_launch_depth_rule():
-    """LBMX_LAUNCH_AUTO streams rows once the lattice gives k_stream 320
-    workgroups (400 for the four-step FP32-storage kernels; FP32 eight steps from
-    6 M cells), and runs the tile kernels on smaller lattices; lattices with walls
+    """LBMX_LAUNCH_AUTO streams rows once the lattice gives k_stream 200
+    workgroups of 32 rows (400 with FP64 collision on FP32 storage; FP32 eight
+    steps from 6 M cells), and runs the tile kernels on smaller lattices; lattices with walls
     take one step per launch."""
     from lbc_amd import native
     cases = [(2048, 2048, 0, 4), (256, 256, 0, 3), (1024, 1024, 0, 4), (2048, 2048, 1, 4),
              (1024, 1024, 1, 4), (256, 256, 1, 3), (2048, 2048, 2, 4), (768, 768, 0, 4),
-             (704, 704, 0, 3), (768, 768, 2, 3), (896, 896, 2, 4), (768, 768, 1, 3),
+             (576, 576, 0, 3), (704, 704, 0, 4), (768, 768, 2, 3), (896, 896, 2, 4),
+             (768, 768, 1, 4), (576, 576, 1, 3),
              (3072, 3072, 1, 8)]
     for Nx, Ny, prec, depth in cases:
         lat = native.Lattice(Nx, Ny, precision=prec, per_x=1, per_y=1)
