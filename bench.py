@@ -12,16 +12,16 @@ levels' rows kept in LDS), bit-identical to one launch per step;
 LBMX_NO_TEMPORAL=1 switches that off.
 
 Also reported in the same JSON line:
-  roofline      algorithmic bytes of one launch of the dominant kernel (144 B per
-                cell-update in FP64: 9 populations read + 9 written, times the
-                cell-updates of the launch) / its duration, from HIP events on the
-                lattice's stream (multi-step launches: the difference of two runs
-                of D*m + 1 steps), vs the 8 TB/s HBM peak; ``traffic`` is the
-                measured HBM bytes per launch from the rocprofv3 FETCH_SIZE /
-                WRITE_SIZE summary committed under profiles/.  A D-step launch
-                moves less than D times the per-step algorithmic bytes, so its
-                fraction can exceed 1;
-                ``secondary.headline_kernel`` has the measured bandwidth
+  roofline      one-pass bytes of one launch of the dominant kernel (144 B per
+                cell in FP64: 9 populations read + 9 written once, the least HBM
+                traffic of a launch that advances every cell D steps) / its
+                duration, from HIP events on the lattice's stream (the difference
+                of two runs of D*m steps), vs the 8 TB/s HBM peak; ``traffic`` is
+                the measured HBM bytes per launch from the rocprofv3 FETCH_SIZE /
+                WRITE_SIZE summary committed under profiles/, ``limiter`` what
+                the rocprofv3 SQ counters say holds the launch back;
+                ``secondary.headline_kernel`` has the per-step algorithmic rate
+                (D times the one-pass bytes per launch) and the counters
   cpu_baseline  the reference library (oracle/_ref, compiled from lbc's own
                 sources with its flags) timed on this host through main_lbm on a
                 bounded sample of the same workload
@@ -96,28 +96,35 @@ def roofline_sweep(sizes=(1024, 2048, 4096, 8192, 16384), seconds=2.0):
     """Periodic Taylor-Green step at each N in FP64 and FP32 storage (FP32
     collision), each point timed for at least `seconds`; moments are not read
     inside the timed region.  Points whose two population buffers fit the 256 MB
-    Infinity Cache are labelled: their rate is not an HBM rate.  The bandwidth is
-    the per-step algorithmic one (144 / 72 B per cell-update); multi-step launches
-    move fewer bytes, so it can exceed the HBM peak."""
+    Infinity Cache are labelled: their rate is not an HBM rate.  Two bandwidths:
+    the one-pass rate (144 / 72 B per cell per launch: a D-step launch reads and
+    writes the lattice once, the least HBM traffic it can have) against the HBM
+    peak, and the per-step algorithmic rate (those bytes per cell-update), which
+    exceeds the peak once a launch takes several steps."""
     from lbc_amd import native
     rows = []
     for N in sizes:
         for name, prec, bpc in (("fp64", native.F64, 144.0), ("fp32", native.F32_SHIFTED, 72.0)):
             lat = tgv_lattice(N, prec)
             _, probe_ms = time_lattice(lat, 20, 10)
+            depth = lat.launch_depth()
             steps = max(50, int(math.ceil(seconds * 1e3 / max(probe_ms / 20, 1e-6))))
+            steps = (steps + depth - 1) // depth * depth
             wall, ms = time_lattice(lat, steps, 10)
             lat.close()
             cells = N * N
             gbs = bpc * cells / (ms / steps * 1e-3) / 1e9
-            rows.append({"N": N, "precision": name, "steps": steps,
+            one_pass = gbs / depth
+            rows.append({"N": N, "precision": name, "steps": steps, "steps_per_launch": depth,
                          "MLUPS": round(cells * steps / wall / 1e6, 1),
-                         "alg_GB_per_s": round(gbs, 1),
-                         "alg_frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                         "one_pass_GB_per_s": round(one_pass, 1),
+                         "one_pass_frac_hbm_peak": round(one_pass / HBM_PEAK_GBS, 4),
+                         "per_step_alg_GB_per_s": round(gbs, 1),
                          # two buffers of bpc / 2 bytes per cell
                          "cache_resident": bool(bpc * cells <= INFINITY_CACHE_BYTES)})
-            log(f"sweep {name} {N}^2: {rows[-1]['MLUPS']:.0f} MLUPS, {gbs:.0f} GB/s "
-                f"({steps} steps{', cache resident' if rows[-1]['cache_resident'] else ''})")
+            log(f"sweep {name} {N}^2: {rows[-1]['MLUPS']:.0f} MLUPS, one pass {one_pass:.0f} "
+                f"GB/s at {depth} steps per launch ({steps} steps"
+                f"{', cache resident' if rows[-1]['cache_resident'] else ''})")
     return {"workload": "taylor_green_periodic_NxN_roofline_sweep", "points": rows}
 
 
@@ -503,41 +510,84 @@ def load_traffic(tag):
         return None
 
 
-def cpu_baseline(N, min_seconds=10.0, max_steps=200):
-    """The reference library's main_lbm on a bounded sample of the TGV workload."""
-    import oracle as O
-    ref = O.reference()
-    kind = "reference"
-    lat = O.Lattice(N, N)
-    k = 2.0 * math.pi / N
-    j, i = np.mgrid[0:N, 0:N].astype(np.float64)
-    u0 = 0.01
-    lat.u[:] = u0 * np.sin(k * i) * np.cos(k * j)
-    lat.v[:] = -u0 * np.cos(k * i) * np.sin(k * j)
-    lat.rho[:] = 1.0 - 0.75 * u0 * u0 * (np.cos(2 * k * i) + np.cos(2 * k * j))
-    from lbc_amd.cases import equilibrium
-    lat.f[:] = equilibrium(lat.rho, lat.u, lat.v)
-    lat.f_new[:] = lat.f
-    lat.omega[:] = 1.0 / 0.8
-    bcs = [("periodic_x", 0, None, None), ("periodic_y", 0, None, None)]
-    if ref is None:
-        kind = "port"
-        step = lambda: O.oracle_step(lat, bcs, 1, 1)   # noqa: E731
-    else:
-        step = lambda: ref.step(lat, bcs, 1, 1)        # noqa: E731
-    step()
-    n, t0 = 0, time.perf_counter()
-    while n < max_steps:
-        step()
-        n += 1
-        if time.perf_counter() - t0 >= min_seconds and n >= 3:
-            break
-    dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(N * N * n / dt / 1e6, 2), "unit": "MLUPS", "cores": threads,
-            "kind": kind,
-            "sample": f"main_lbm FP64 periodic Taylor-Green {N}x{N}, {n} steps "
-                      f"({dt:.1f} s), OMP threads={threads}"}
+def load_issue(tag):
+    """VALU / LDS issue counters of the fused kernel from the committed rocprofv3
+    SQ-counter summary (profiles/sq_<tag>.json), or None."""
+    p = os.path.join(ROOT, "profiles", f"sq_{tag}.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as fh:
+            return json.load(fh)
+    except Exception:
+        return None
+
+
+def host_cpu():
+    """Model, sockets and physical cores of this host (lscpu), and the CPUs this
+    process may run on."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    def num(key):
+        try:
+            return int(info.get(key, ""))
+        except ValueError:
+            return None
+    sockets, cps, tpc = num("Socket(s)"), num("Core(s) per socket"), num("Thread(s) per core")
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except Exception:
+        allowed = os.cpu_count()
+    return {"model": info.get("Model name"), "sockets": sockets,
+            "physical_cores": sockets * cps if sockets and cps else None,
+            "threads_per_core": tpc, "logical_cpus": num("CPU(s)"),
+            "cpus_allowed": allowed}
+
+
+def cpu_baseline(N, min_seconds=10.0, one_thread_seconds=8.0):
+    """The reference library's main_lbm on a bounded sample of the TGV workload
+    (oracle/cpu_bench.py, a child process per point so that the OpenMP binding takes
+    effect): the box's OpenMP thread count (OMP_NUM_THREADS, the host share this
+    process is given) bound close to one thread per physical core, and one thread."""
+    import subprocess
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0) or 0) or (os.cpu_count() or 1)
+    host = host_cpu()
+    if host["physical_cores"]:
+        threads = min(threads, host["physical_cores"])
+
+    def point(nthreads, seconds):
+        env = dict(os.environ, OMP_NUM_THREADS=str(nthreads), OMP_PROC_BIND="close",
+                   OMP_PLACES="cores")
+        r = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", "--size", str(N),
+                            "--seconds", str(seconds)], cwd=ROOT, env=env,
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"cpu_bench failed ({r.returncode}): {r.stderr[-500:]}")
+        return json.loads(r.stdout.strip().splitlines()[-1])
+
+    full = point(threads, min_seconds)
+    one = point(1, one_thread_seconds)
+    how = ("built from lbc's own src/ with its flags (gcc -O3 -fopenmp) by oracle/Makefile in "
+           "the build container and carried to this host with the repository snapshot"
+           if full["kind"] == "reference" else "this project's C restatement of the step")
+    return {"value": round(full["MLUPS"], 2), "unit": "MLUPS", "cores": threads,
+            "kind": full["kind"],
+            "sample": f"main_lbm FP64 periodic Taylor-Green {N}x{N}, {full['steps']} steps "
+                      f"({full['seconds']:.1f} s) on {threads} OpenMP threads "
+                      f"(OMP_PROC_BIND=close OMP_PLACES=cores); one thread: "
+                      f"{one['steps']} steps ({one['seconds']:.1f} s)",
+            "one_thread_MLUPS": round(one["MLUPS"], 2),
+            "scaling_vs_one_thread": round(full["MLUPS"] / one["MLUPS"], 2),
+            "binary": f"{full['binary']} ({how})",
+            "binding": "OMP_PROC_BIND=close OMP_PLACES=cores",
+            "host": host}
 
 
 def main():
@@ -596,15 +646,16 @@ def main():
         "f32wide": (native.F32_WIDE, 72, "fp32wide", "FP32-storage FP64-collision")}[args.precision]
     lat = tgv_lattice(N, prec)
     wall, dev_ms = time_lattice(lat, args.steps, args.warmup)
-    # the dominant kernel and its launch time: a run of D*m + 1 steps is m D-step
-    # launches and one one-step launch (k_pull), so the difference of two such runs
-    # is a whole number of D-step launches
+    # the dominant kernel and its launch time: once the lattice runs in gather form a
+    # run of D*m steps is exactly m D-step launches, so the difference of two such
+    # runs over the difference of m is one launch, from HIP events on the lattice's
+    # stream
     steps_per_launch = lat.launch_depth()
     temporal = steps_per_launch > 1
     if temporal:
         m_a, m_b = 40, 10
-        t_a = lat.time_steps(steps_per_launch * m_a + 1)
-        t_b = lat.time_steps(steps_per_launch * m_b + 1)
+        t_a = lat.time_steps(steps_per_launch * m_a)
+        t_b = lat.time_steps(steps_per_launch * m_b)
         launch_ms = (t_a - t_b) / (m_a - m_b)
     else:
         launch_ms = dev_ms / args.steps
@@ -612,12 +663,19 @@ def main():
               if prec == native.F64 else f"{pname} gather form, {steps_per_launch} steps per launch")
     lat.close()
     mlups = cells * args.steps / wall / 1e6
-    achieved = float(bpc) * cells * steps_per_launch / (launch_ms * 1e-3) / 1e9
-    traffic = load_traffic(f"tgv{N}_{ptag}" + (f"_{steps_per_launch}step" if temporal else ""))
+    # roofline: the least HBM traffic of one launch is one read and one write of the
+    # lattice (bpc B per cell), whatever its steps per launch; per-step algorithmic
+    # bytes (bpc per cell-update) go in secondary, since a D-step launch moves only
+    # a D-th of them
+    achieved = float(bpc) * cells / (launch_ms * 1e-3) / 1e9
+    per_step_alg = achieved * steps_per_launch
+    ptag_launch = f"tgv{N}_{ptag}" + (f"_{steps_per_launch}step" if temporal else "")
+    traffic = load_traffic(ptag_launch)
+    issue = load_issue(ptag_launch)
     log(f"{pname} {N}^2: {wall / args.steps * 1e3:.3f} ms/step wall, "
         f"{dev_ms / args.steps:.3f} ms/step events, {mlups:.0f} MLUPS; {kernel}: "
-        f"{launch_ms:.3f} ms per launch, {achieved:.0f} GB/s algorithmic ({bpc} B per "
-        f"cell-update)")
+        f"{launch_ms:.3f} ms per launch, one pass {achieved:.0f} GB/s "
+        f"({achieved / HBM_PEAK_GBS:.1%} of HBM peak); per-step algorithmic {per_step_alg:.0f} GB/s")
 
     secondary = {}
     if not args.no_secondary:
@@ -629,7 +687,8 @@ def main():
         secondary["fp32_shifted_tgv"] = {
             "workload": f"taylor_green_periodic_{N}x{N}_fp32_storage_fp32_collision",
             "MLUPS": round(m32, 1),
-            "alg_GB_per_s": round(72.0 * cells / (d32 / args.steps * 1e-3) / 1e9, 1),
+            "one_pass_GB_per_s": round(72.0 * cells / (d32 / args.steps * spl32 * 1e-3) / 1e9, 1),
+            "per_step_alg_GB_per_s": round(72.0 * cells / (d32 / args.steps * 1e-3) / 1e9, 1),
             "steps_per_launch": spl32,
             "vs_fp64": round(m32 / mlups, 3)}
         log(f"FP32-shifted {N}^2: {m32:.0f} MLUPS ({m32 / mlups:.2f}x FP64)")
@@ -641,7 +700,8 @@ def main():
         secondary["fp32_wide_tgv"] = {
             "workload": f"taylor_green_periodic_{N}x{N}_fp32_storage_fp64_collision",
             "MLUPS": round(mw, 1),
-            "alg_GB_per_s": round(72.0 * cells / (dw / args.steps * 1e-3) / 1e9, 1),
+            "one_pass_GB_per_s": round(72.0 * cells / (dw / args.steps * splw * 1e-3) / 1e9, 1),
+            "per_step_alg_GB_per_s": round(72.0 * cells / (dw / args.steps * 1e-3) / 1e9, 1),
             "steps_per_launch": splw,
             "vs_fp64": round(mw / mlups, 3)}
         log(f"FP32 storage, FP64 collision {N}^2: {mw:.0f} MLUPS ({mw / mlups:.2f}x FP64)")
@@ -678,12 +738,16 @@ def main():
     secondary["headline_kernel"] = {
         "kernel": kernel, "steps_per_launch": steps_per_launch,
         "launch_ms": round(launch_ms, 4),
-        "algorithmic_bytes_per_launch": bpc * cells * steps_per_launch,
+        "one_pass_bytes_per_launch": bpc * cells,
+        "per_step_algorithmic_bytes_per_launch": bpc * cells * steps_per_launch,
+        "per_step_alg_GB_per_s": round(per_step_alg, 1),
         "hbm_bytes_per_launch": traffic,
         "hbm_GB_per_s": round(traffic / (launch_ms * 1e-3) / 1e9, 1) if traffic else None,
-        "note": f"{steps_per_launch} steps per HBM pass move less than the per-step "
-                "algorithmic bytes, so the per-step roofline fraction can exceed 1"
-                if temporal else ""}
+        "issue": issue,
+        "note": f"{steps_per_launch} steps per HBM pass: the roofline line counts one read "
+                "and one write of the lattice per launch; the launch also runs "
+                f"{steps_per_launch} collisions per cell, and 'issue' gives the VALU and LDS "
+                "counters of the same launch" if temporal else ""}
     out = {
         "metric": f"MLUPS ({pname} fused D2Q9 BGK step) + % HBM roofline",
         "value": round(mlups, 1),
@@ -701,7 +765,8 @@ def main():
                    "tau": 0.8, "u0": 0.01, "lattice": "D2Q9 BGK", "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic,
+                     "limiter": (issue or {}).get("limiter")},
         "cpu_baseline": cpu,
         "secondary": secondary,
     }

@@ -146,7 +146,9 @@ LBMX_API int lbmx_step(lbmx_lattice *lat, int nsteps);
  *   LBMX_LAUNCH_STREAM  4 (FP64, FP32 storage with FP64 collision) or 8 (FP32)
  *                       steps per launch, each workgroup streaming down a column
  *                       strip with one wavefront per time level
- * Every run ends with a one-step launch.  Other lattices ignore the mode. */
+ * A run of n steps that enters this form spends its first step on an in-place
+ * collision; the remaining steps go in launches of the mode's depth, the last
+ * few in shallower ones.  Other lattices ignore the mode. */
 #define LBMX_LAUNCH_AUTO 0
 #define LBMX_LAUNCH_SINGLE 1
 #define LBMX_LAUNCH_TILES 2

@@ -190,6 +190,13 @@ LBMX_API int lbmx_server_stats(lbmx_server *s, int *waiting, int *running, int *
 /* Stops the worker after the running and waiting requests finish, then frees. */
 LBMX_API void lbmx_server_destroy(lbmx_server *s);
 
+/* The server's spectrum and TL (the same device reduction, k_spectrum) of a
+ * completed lattice of a batch: out receives nfreq, tl_db, up_abs and down_abs
+ * (tag -1, times 0).  A request's result from the server equals this for the
+ * same case run alone in a batch. */
+LBMX_API int lbmx_batch_spectrum(lbmx_batch *b, int id, const lbmx_tl_spec *tl,
+                                 lbmx_result *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,7 @@ def _bind():
                                     C.POINTER(sz)]
     L.lbmx_server_destroy.argtypes = [vp]
     L.lbmx_server_destroy.restype = None
+    L.lbmx_batch_spectrum.argtypes = [vp, i, C.POINTER(TlSpec), C.POINTER(Result)]
     _bound = True
     return L
 
@@ -400,6 +401,12 @@ class Batch:
         if n < 0:
             check(n)
         return h[:n, :len(c.probes)]
+
+    def spectrum(self, i, tl: TlSpec) -> Result:
+        """The server's device spectrum and TL of completed lattice i."""
+        r = Result()
+        check(self._L.lbmx_batch_spectrum(self.h, i, C.byref(tl), C.byref(r)))
+        return r
 
     def tune(self, trial_steps=4):
         t = Tuning()

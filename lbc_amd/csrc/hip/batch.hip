@@ -498,6 +498,50 @@ const BatchSlot *lbmx_batch_slot(const lbmx_batch *b, int id)
     return &b->slots[id];
 }
 
+int lbmx_tl_check(const lbmx_case *c, const lbmx_tl_spec *tl)
+{
+    if (tl->nfreq < 1 || tl->nfreq > 64) return lbmx_fail(LBMX_ERR_ARG, "nfreq must be in [1, 64]");
+    if (tl->up_probe < 0 || tl->up_probe >= c->nprobe || tl->down_probe < 0 ||
+        tl->down_probe >= c->nprobe)
+        return lbmx_fail(LBMX_ERR_ARG, "spectrum probes must name probes of the case");
+    if (tl->up_t0 < 0 || tl->up_t1 <= tl->up_t0 || tl->up_t1 > c->steps + 1 ||
+        tl->down_t0 < 0 || tl->down_t1 <= tl->down_t0 || tl->down_t1 > c->steps + 1)
+        return lbmx_fail(LBMX_ERR_ARG, "probe windows must lie inside [0, steps]");
+    return LBMX_OK;
+}
+
+int lbmx_batch_spectrum_of(const BatchSlot &S, const lbmx_tl_spec &tl, hipStream_t stream,
+                           double *dev, double *host, lbmx_result *out)
+{
+    SpecArgs a{};
+    a.hist = S.d.probe_hist;
+    a.nprobe = S.c.nprobe;
+    a.probe[0] = tl.up_probe;
+    a.probe[1] = tl.down_probe;
+    a.t0[0] = tl.up_t0;
+    a.t1[0] = tl.up_t1;
+    a.t0[1] = tl.down_t0;
+    a.t1[1] = tl.down_t1;
+    a.rho0 = S.c.rho0;
+    const double cs = 1.0 / std::sqrt(3.0);
+    a.cs2 = cs * cs;
+    a.f_lo = tl.f_lo;
+    a.df = tl.nfreq > 1 ? (tl.f_hi - tl.f_lo) / (tl.nfreq - 1) : 0.0;
+    a.out = dev;
+    hipLaunchKernelGGL(k_spectrum, dim3(tl.nfreq, 2), dim3(kSpecThreads), 0, stream, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host, dev, (size_t)tl.nfreq * 4 * 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    out->nfreq = tl.nfreq;
+    for (int f = 0; f < tl.nfreq; f++) {
+        const double *v = host + 4 * f;
+        out->up_abs[f] = std::hypot(v[0], v[1]);
+        out->down_abs[f] = std::hypot(v[2], v[3]);
+        out->tl_db[f] = 20.0 * std::log10(out->up_abs[f] / out->down_abs[f]);
+    }
+    return LBMX_OK;
+}
+
 // ---- C ABI --------------------------------------------------------------------------
 extern "C" {
 
@@ -577,6 +621,8 @@ LBMX_API void lbmx_batch_destroy(lbmx_batch *b)
     }
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->spec_dev) hipFree(b->spec_dev);
+    if (b->spec_host) hipHostFree(b->spec_host);
     if (b->lats_dev) hipFree(b->lats_dev);
     if (b->tiles_dev) hipFree(b->tiles_dev);
     if (b->ctl_dev) hipFree(b->ctl_dev);
@@ -721,6 +767,25 @@ LBMX_API int lbmx_batch_probes(lbmx_batch *b, int id, double *hist, int capacity
     HIPCHK(hipMemcpy(hist, S->d.probe_hist, (size_t)rows * S->c.nprobe * 8,
                      hipMemcpyDeviceToHost));
     return rows;
+}
+
+LBMX_API int lbmx_batch_spectrum(lbmx_batch *b, int id, const lbmx_tl_spec *tl,
+                                 lbmx_result *out)
+{
+    const BatchSlot *S = lbmx_batch_slot(b, id);
+    if (!S || !tl || !out) return lbmx_fail(LBMX_ERR_ARG, "invalid argument");
+    if (int rc = lbmx_tl_check(&S->c, tl)) return rc;
+    int done = 0;
+    lbmx_batch_steps_done(b, id, &done);
+    if (done < S->c.steps) return lbmx_fail(LBMX_ERR_STATE, "lattice %d has not completed", id);
+    if (!b->spec_dev) {
+        HIPCHK(hipMalloc((void **)&b->spec_dev, 64 * 4 * 8));
+        HIPCHK(hipHostMalloc((void **)&b->spec_host, 64 * 4 * 8, hipHostMallocDefault));
+    }
+    *out = lbmx_result{};
+    out->tag = -1;
+    out->steps = S->c.steps;
+    return lbmx_batch_spectrum_of(*S, *tl, b->stream, b->spec_dev, b->spec_host, out);
 }
 
 LBMX_API int lbmx_batch_set_tuning(lbmx_batch *b, const lbmx_tuning *t)

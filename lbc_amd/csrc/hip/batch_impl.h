@@ -74,7 +74,15 @@ struct lbmx_batch {
     std::vector<BatchStaging> ring;
     int ring_next = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // lbmx_batch_spectrum's buffers (allocated on first use)
+    double *spec_dev = nullptr, *spec_host = nullptr;
 };
 
 int lbmx_batch_enqueue(lbmx_batch *b, int nsteps);
 const BatchSlot *lbmx_batch_slot(const lbmx_batch *b, int id);
+// Checks a spectrum request against the case's probes and length.
+int lbmx_tl_check(const lbmx_case *c, const lbmx_tl_spec *tl);
+// Probe spectra of a lattice's history (k_spectrum on `stream`, through the device
+// buffer dev and the pinned host buffer host, 4 * 64 doubles each) and the TL.
+int lbmx_batch_spectrum_of(const BatchSlot &S, const lbmx_tl_spec &tl, hipStream_t stream,
+                           double *dev, double *host, lbmx_result *out);

@@ -674,9 +674,8 @@ void lbmx_reset_run(lbmx_lattice *L)
 extern "C" int lbmx_set_force(lbmx_lattice *L, double Fx, double Fy)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
-    // in gather form the next step's collision has already used the old force
-    if (L->pull && (Fx != L->cfg.Fx || Fy != L->cfg.Fy))
-        if (int prc = lbmx_ensure_push(L)) return prc;
+    // (in gather form the stored field is the last step's post-collision one, so
+    // the next collision has not run yet and takes the new force)
     L->cfg.Fx = Fx;
     L->cfg.Fy = Fy;
     return LBMX_OK;
@@ -1172,6 +1171,10 @@ static PullArgs pull_args(const lbmx_lattice *L)
     return a;
 }
 
+// Entering the gather form is the collision of the next step, in place: on a
+// periodic all-fluid lattice the step's streaming is the permutation the next
+// gather applies, so the buffer then holds that step's post-collision field (the
+// reference's f_new after the driver's swap) and the step counts as done.
 template <typename T>
 static int enter_pull(lbmx_lattice *L)
 {
@@ -1183,6 +1186,10 @@ static int enter_pull(lbmx_lattice *L)
     L->pull = 1;
     L->pstar = L->cur;
     L->lag_arrays = 0;
+    L->last = a.s;
+    L->Fx_lag = L->cfg.Fx;
+    L->Fy_lag = L->cfg.Fy;
+    L->steps++;
     return LBMX_OK;
 }
 
@@ -1208,8 +1215,8 @@ static int pull_step(lbmx_lattice *L)
     return LBMX_OK;
 }
 
-// Two gather-form steps in one launch (k_pull2); the buffer of f*_{s+1} is not
-// written, so callers end every run of steps with pull_step.
+// Two gather-form steps in one launch (k_pull2); f*_{s+1} stays in LDS and only
+// f*_{s+2} is stored.
 template <typename T, bool W, int TY>
 static void launch_pull2(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
 {
@@ -1494,16 +1501,18 @@ static int pull_step2(lbmx_lattice *L)
 int lbmx_ensure_push(lbmx_lattice *L)
 {
     if (!L->pull) return LBMX_OK;
+    // pop[pstar] holds f*_{k-1}, the last step's post-collision field: its gather
+    // is the newest pre-collision state f_k, and it stays as the reference's f_new
     const Geo g = lbmx_geo(L);
     dim3 grid((L->Nx + 255) / 256, L->Ny);
     if (L->esize == 8)
         hipLaunchKernelGGL(k_gather<double>, grid, dim3(256), 0, L->stream,
-                           (const double *)L->pop[L->pstar ^ 1], (double *)L->pop[L->pstar], g);
+                           (const double *)L->pop[L->pstar], (double *)L->pop[L->pstar ^ 1], g);
     else
         hipLaunchKernelGGL(k_gather<float>, grid, dim3(256), 0, L->stream,
-                           (const float *)L->pop[L->pstar ^ 1], (float *)L->pop[L->pstar], g);
+                           (const float *)L->pop[L->pstar], (float *)L->pop[L->pstar ^ 1], g);
     HIPCHK(hipGetLastError());
-    L->cur = L->pstar;
+    L->cur = L->pstar ^ 1;
     L->pull = 0;
     L->other_is_ref = 1;
     L->lag_arrays = 0;
@@ -1537,6 +1546,7 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
             return lbmx_fail(LBMX_ERR_STATE, "boundary %zu has no values", b);
     if (nsteps <= 0) return LBMX_OK;
     const bool f64 = (L->esize == 8);
+    const int64_t steps0 = L->steps;
     if (!L->pull && nsteps >= 2 && pull_eligible(L)) {
         int rc = f64 ? enter_pull<double>(L) : enter_pull<float>(L);
         if (rc) return rc;
@@ -1545,27 +1555,28 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
         int rc = lbmx_ensure_push(L);
         if (rc) return rc;
     }
-    int i = 0;
+    int i = L->steps - steps0;   // 1 when entering the gather form took the first step
     // tile depth: 3 steps per launch (the tiles serve the lattices too small for
     // row streaming, where three beat two in FP64 as in FP32: 256^2 3.69 vs 3.97
     // us per step, 512^2 4.66 vs 5.05); LBMX_TEMPORAL_DEPTH=2 or 3 overrides
     const int depth = tile_depth();
     const int mode = L->launch_mode;
     if (L->pull && mode != LBMX_LAUNCH_SINGLE) {
-        // multi-step launches while more steps remain than they take: the run ends
-        // with pull_step
+        // multi-step launches while at least as many steps remain as they take; the
+        // buffer a launch writes is the post-collision field of its last step, all a
+        // following run or the way back to the reference's buffers needs
         const int sd = stream_depth(L);
         if (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && stream_fits(L)))
-            for (; nsteps - i >= sd + 1; i += sd) {
+            for (; nsteps - i >= sd; i += sd) {
                 int rc = f64 ? pull_stream<double>(L) : pull_stream<float>(L);
                 if (rc) return rc;
             }
         if (depth >= 3)
-            for (; nsteps - i >= 4; i += 3) {
+            for (; nsteps - i >= 3; i += 3) {
                 int rc = f64 ? pull_step3<double>(L) : pull_step3<float>(L);
                 if (rc) return rc;
             }
-        for (; nsteps - i >= 3; i += 2) {
+        for (; nsteps - i >= 2; i += 2) {
             int rc = f64 ? pull_step2<double>(L) : pull_step2<float>(L);
             if (rc) return rc;
         }
@@ -1888,8 +1899,8 @@ int lbmx_launch_moments(lbmx_lattice *L)
     MomArgs m;
     m.g = lbmx_geo(L);
     m.F = Fields{L->Ro, L->Uo, L->Vo};
-    m.Fx = L->cfg.Fx;
-    m.Fy = L->cfg.Fy;
+    m.Fx = L->Fx_lag;   // the force of the last step, as its compute_macroscopic_fields took it
+    m.Fy = L->Fy_lag;
     dim3 grid((L->Nx + 255) / 256, L->Ny);
     if (L->esize == 8)
         hipLaunchKernelGGL(k_moments<double>, grid, dim3(256), 0, L->stream,

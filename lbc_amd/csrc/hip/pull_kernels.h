@@ -11,11 +11,14 @@
 // Rounding is identical to the push path: the same moments_ref / collide_ref
 // (FP64) or moments_f32 / collide_f32 (FP32-shifted) run on the same values.
 //
+// State: the buffer pop[pstar] holds f*_{k-1}, the post-collision field of the
+// last step done (the reference's f_new after the driver's swap; a periodic
+// all-fluid lattice has no stale entry), and f_k = gather(f*_{k-1}).
 // Conversions (bit-exact):
-//   to gather form   k_collide_inplace: f*_k = collide(f_k) in the current buffer
+//   to gather form   k_collide_inplace: f*_k = collide(f_k) in the current buffer,
+//                    which is the next step (its streaming is the next gather)
 //   back to the reference's buffers   k_gather: f_k = gather(f*_{k-1}) into the
-//                    buffer that held f*_k; the other buffer keeps f*_{k-1},
-//                    which is exactly the reference's f_new after the swap.
+//                    other buffer; pop[pstar] stays as the reference's f_new.
 #pragma once
 #include <utility>
 #include "step_kernels.h"
@@ -126,8 +129,7 @@ __global__ void __launch_bounds__(256) k_collide_inplace(T *__restrict__ S, Pull
 // writes f*_{s+2} once: about 60 % of the HBM traffic of two k_pull launches for
 // the same two steps.  The arithmetic is k_pull's, on the same values, so the
 // result is bit-identical to two k_pull launches.  The intermediate f*_{s+1} is
-// never stored, so a run of steps ends with a k_pull step (the buffer pair then
-// holds f*_{k-1} and f*_k, as lbmx_ensure_push needs).
+// never stored; f*_{s+2} is all a following step or lbmx_ensure_push needs.
 template <typename T, bool W>
 __device__ __forceinline__ void pull_collide(T p[Q], const Geo &g, double Fx_lag, double Fy_lag,
                                              bool force, double Fx, double Fy)

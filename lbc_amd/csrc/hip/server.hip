@@ -72,52 +72,12 @@ struct lbmx_server {
     }
 };
 
-static int validate_tl(const lbmx_case *c, const lbmx_tl_spec *tl)
-{
-    if (tl->nfreq < 1 || tl->nfreq > 64) return lbmx_fail(LBMX_ERR_ARG, "nfreq must be in [1, 64]");
-    if (tl->up_probe < 0 || tl->up_probe >= c->nprobe || tl->down_probe < 0 ||
-        tl->down_probe >= c->nprobe)
-        return lbmx_fail(LBMX_ERR_ARG, "spectrum probes must name probes of the case");
-    if (tl->up_t0 < 0 || tl->up_t1 <= tl->up_t0 || tl->up_t1 > c->steps + 1 ||
-        tl->down_t0 < 0 || tl->down_t1 <= tl->down_t0 || tl->down_t1 > c->steps + 1)
-        return lbmx_fail(LBMX_ERR_ARG, "probe windows must lie inside [0, steps]");
-    return LBMX_OK;
-}
-
 // Spectra of one finished lattice (aux stream), then TL on the host.
 static int finish(lbmx_server *s, const Running &R, lbmx_result *out)
 {
     const BatchSlot *S = lbmx_batch_slot(s->b, R.id);
     if (!S) return lbmx_fail(LBMX_ERR_STATE, "finished lattice vanished");
-    const lbmx_tl_spec &tl = R.r.tl;
-    SpecArgs a{};
-    a.hist = S->d.probe_hist;
-    a.nprobe = S->c.nprobe;
-    a.probe[0] = tl.up_probe;
-    a.probe[1] = tl.down_probe;
-    a.t0[0] = tl.up_t0;
-    a.t1[0] = tl.up_t1;
-    a.t0[1] = tl.down_t0;
-    a.t1[1] = tl.down_t1;
-    a.rho0 = S->c.rho0;
-    const double cs = 1.0 / std::sqrt(3.0);
-    a.cs2 = cs * cs;
-    a.f_lo = tl.f_lo;
-    a.df = tl.nfreq > 1 ? (tl.f_hi - tl.f_lo) / (tl.nfreq - 1) : 0.0;
-    a.out = s->spec_dev;
-    hipLaunchKernelGGL(k_spectrum, dim3(tl.nfreq, 2), dim3(kSpecThreads), 0, s->aux, a);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(s->spec_host, s->spec_dev, (size_t)tl.nfreq * 4 * 8,
-                          hipMemcpyDeviceToHost, s->aux));
-    HIPCHK(hipStreamSynchronize(s->aux));
-    out->nfreq = tl.nfreq;
-    for (int f = 0; f < tl.nfreq; f++) {
-        const double *v = s->spec_host + 4 * f;
-        out->up_abs[f] = std::hypot(v[0], v[1]);
-        out->down_abs[f] = std::hypot(v[2], v[3]);
-        out->tl_db[f] = 20.0 * std::log10(out->up_abs[f] / out->down_abs[f]);
-    }
-    return LBMX_OK;
+    return lbmx_batch_spectrum_of(*S, R.r.tl, s->aux, s->spec_dev, s->spec_host, out);
 }
 
 static void publish(lbmx_server *s, const Request &r, int status, double t_admit,
@@ -261,7 +221,7 @@ LBMX_API int lbmx_server_submit(lbmx_server *s, const lbmx_case *c, const lbmx_t
                                 long long tag)
 {
     if (!s || !c || !tl) return lbmx_fail(LBMX_ERR_ARG, "null argument");
-    if (int rc = validate_tl(c, tl)) return rc;
+    if (int rc = lbmx_tl_check(c, tl)) return rc;
     if (c->init == LBMX_INIT_FIELDS)
         return lbmx_fail(LBMX_ERR_ARG, "served requests start from a generated state");
     Request r{*c, *tl, tag, s->now()};

@@ -188,6 +188,8 @@ int stage_lattice(int Nx, int Ny, int per_x, int per_y, int ibb, double Fx, doub
     }
     c.lat->cfg.Fx = Fx;
     c.lat->cfg.Fy = Fy;
+    c.lat->Fx_lag = Fx;   // a stand-alone stage is its own step: moments use its force
+    c.lat->Fy_lag = Fy;
     *out = c.lat;
     return LBMX_OK;
 }

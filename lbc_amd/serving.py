@@ -70,12 +70,16 @@ class LinerRequest:
         return c
 
     def windows(self):
-        """[(t0, t1) upstream, (t0, t1) downstream] sample windows of the spectra."""
+        """[(t0, t1) upstream, (t0, t1) downstream] sample windows of the spectra.
+        A request shorter than min_steps(Nx) ends before the transmitted pulse has
+        passed the downstream probe, so it has no TL: ValueError."""
+        if self.steps < min_steps(self.Nx):
+            raise ValueError(f"a {self.Nx}-wide duct needs >= {min_steps(self.Nx)} steps for "
+                             f"its transmitted pulse (asked for {self.steps})")
         t_u = (self.Nx / 8.0) / CS
         t_d = (5.0 * self.Nx / 8.0) / CS
-        up = (0, int(min(self.steps, round(2.0 * t_u))))
-        dn0 = int(min(self.steps - 1, round(t_d - t_u)))
-        dn = (dn0, int(max(dn0 + 1, min(self.steps, round(t_d + t_u)))))
+        up = (0, int(round(2.0 * t_u)))
+        dn = (int(round(t_d - t_u)), int(round(t_d + t_u)))
         return up, dn
 
     def tl_spec(self) -> B.TlSpec:
@@ -83,14 +87,22 @@ class LinerRequest:
         return B.TlSpec(NFREQ, F_LO, F_HI, 0, 1, u0, u1, d0, d1)
 
 
+def min_steps(Nx):
+    """Shortest run whose transmitted pulse (from x = Nx/8) has passed the
+    downstream probe's window (x = 3 Nx/4, half-width Nx/8 / cs): 3 Nx / (4 cs)."""
+    return int(math.ceil(0.75 * Nx / CS))
+
+
 def draw_requests(n, seed=7, Nx=(1024, 8192), Ny=(64, 256), R=(0.5, 3.0),
                   f_res=(1.0 / 80.0, 1.0 / 20.0), amp=(1e-4, 1e-3), steps=(2000, 20000),
                   M=(1.0, 16.0)):
-    """n seeded requests: Nx, Ny uniform (multiples of 32); R, f_res, mass,
-    amplitude and length uniform in the given ranges.  Every draw is stable under
-    the reference's ODE step: it updates chi with the new psi (nit.c:31), which
-    makes the pair's amplification matrix [[1+a, -b], [b(1+a), 1+a-b^2]] with
-    determinant (1+a)^2 < 1 for the damped poles a +- ib."""
+    """n seeded requests: Nx, Ny uniform (multiples of 32); R, f_res, mass and
+    amplitude uniform in the given ranges; the length uniform in the given range
+    above min_steps(Nx) (a shorter run has no transmitted pulse to measure, so
+    such a draw is drawn again).  Every draw is stable under the reference's ODE
+    step: it updates chi with the new psi (nit.c:31), which makes the pair's
+    amplification matrix [[1+a, -b], [b(1+a), 1+a-b^2]] with determinant
+    (1+a)^2 < 1 for the damped poles a +- ib."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
@@ -100,7 +112,11 @@ def draw_requests(n, seed=7, Nx=(1024, 8192), Ny=(64, 256), R=(0.5, 3.0),
         fr = float(rng.uniform(*f_res))
         m = float(rng.uniform(*M))
         a = float(rng.uniform(*amp))
+        if min_steps(nx) > steps[1]:
+            raise ValueError(f"steps up to {steps[1]} cannot cover a {nx}-wide duct")
         st = int(rng.integers(steps[0], steps[1] + 1))
+        while st < min_steps(nx):
+            st = int(rng.integers(steps[0], steps[1] + 1))
         out.append(LinerRequest(nx, ny, r, fr, m, a, st))
     return out
 

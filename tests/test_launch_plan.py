@@ -15,7 +15,8 @@ from lbc_amd import native
     (576, 576, native.F32_SHIFTED, 3), (640, 640, native.F32_SHIFTED, 4),
     (16384, 16384, native.F32_SHIFTED, 8), (2048, 2048, native.F32_WIDE, 4),
     (8192, 256, native.F64, 4), (64, 100000, native.F64, 4),
-    # around the streaming threshold (320 workgroups; 400 for F32_WIDE)
+    # around the streaming threshold (200 workgroups of 32 rows at four steps, 320
+    # at eight, 400 for F32_WIDE)
     (576, 576, native.F64, 3), (640, 640, native.F64, 4), (704, 704, native.F64, 4),
     (768, 768, native.F64, 4), (768, 768, native.F32_WIDE, 3),
     (896, 896, native.F32_WIDE, 4), (1408, 1408, native.F32_SHIFTED, 4),

@@ -195,14 +195,21 @@ def test_launch_modes_fp32_identical(sc, precision):
             assert same_bits(a, b)
 
 
-def test_fp32_four_step_stream_identical():
-    """FP32 storage and collision at four steps per launch (the default on
-    mid-size lattices, 1000 x 1030 here: 18 strips x 65 segments of 16 rows, the
-    last of each partial) rounds exactly as one step per launch, with a force."""
+@pytest.mark.parametrize("Nx,Ny,plan", [
+    # 18 strips x 65 segments of 16 rows, the last of each partial
+    (1000, 1030, (4, 58, 16)),
+    # 32-row segments (920 workgroups)
+    (1280, 1280, (4, 58, 32)),
+    # a narrow lattice: one full 58-column strip and a partial 6-column one
+    (64, 90000, (4, 58, 32))], ids=["16row", "32row", "narrow"])
+def test_fp32_four_step_stream_identical(Nx, Ny, plan):
+    """FP32 storage and collision at four steps per launch (the automatic choice
+    below 6 M cells) rounds exactly as one step per launch, with a force, for the
+    segment heights and strip shapes the automatic rule reaches."""
     from lbc_amd import native
-    sc = Scenario("tgv_f32_stream4", 1000, 1030, [("periodic_x", 0), ("periodic_y", 0)], 42,
-                  Fx=1e-6, Fy=-2e-6)
-    assert native.launch_plan(sc.Nx, sc.Ny, native.F32_SHIFTED) == (4, 58, 16)
+    sc = Scenario(f"tgv_f32_stream4_{Nx}x{Ny}", Nx, Ny, [("periodic_x", 0), ("periodic_y", 0)],
+                  42, Fx=1e-6, Fy=-2e-6)
+    assert native.launch_plan(sc.Nx, sc.Ny, native.F32_SHIFTED) == plan
     state = sc.initial_state()
     ref = _launch_run(sc, state, (14,), native.LAUNCH_SINGLE, native.F32_SHIFTED)
     got = _launch_run(sc, state, (14,), native.LAUNCH_AUTO, native.F32_SHIFTED)

@@ -25,7 +25,11 @@ def test_request_distribution():
     assert nx.min() >= 1024 and nx.max() <= 8192 and (nx % 32 == 0).all()
     assert ny.min() >= 64 and ny.max() <= 256 and (ny % 32 == 0).all()
     assert st.min() >= 2000 and st.max() <= 20000
-    assert abs(st.mean() - 11000) < 400
+    # long enough for the transmitted pulse: uniform over [max(2000, min_steps), 20000]
+    need = np.array([SV.min_steps(x) for x in nx])
+    assert (st >= need).all()
+    expect = ((np.maximum(2000, need) + 20000) / 2.0).mean()
+    assert abs(st.mean() - expect) < 400
     R = np.array([r.R for r in rs])
     fr = np.array([r.f_res for r in rs])
     amp = np.array([r.amp for r in rs])
@@ -73,6 +77,17 @@ def test_in_place_step_amplification():
     M = np.array([[1 + a, -b], [b * (1 + a), 1 + a - b * b]])
     assert abs(np.linalg.det(M) - (1 + a) ** 2) < 1e-14
     assert euler_growth(imp) < 1e-3
+
+
+def test_short_request_has_no_windows():
+    r = SV.LinerRequest(6208, 160, 2.0, 1 / 25, 14.6, 1.4e-4, 5306)
+    assert r.steps < SV.min_steps(r.Nx)
+    try:
+        r.windows()
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("a request ending before its transmitted pulse has windows")
 
 
 def test_windows_follow_the_pulse():

@@ -148,7 +148,7 @@ def test_budget_admission_and_completion():
 
 
 def test_request_that_cannot_fit_fails_cleanly():
-    big = SV.LinerRequest(8192, 256, 1.0, 1 / 40, 4.0, 5e-4, 100)
+    big = SV.LinerRequest(8192, 256, 1.0, 1 / 40, 4.0, 5e-4, SV.min_steps(8192))
     small = requests(1)[0]
     s = B.Server(PREC, budget=int(1.5 * B.case_bytes(small.case(), PREC)))
     s.submit(big.case(), big.tl_spec(), 0)

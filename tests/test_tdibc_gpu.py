@@ -124,8 +124,10 @@ def test_liner_fp64_bit_identical():
 
 
 # FP32-shifted vs FP64 on the liner: wall pressure and transmitted probe, relative
-# to the pulse scale (amp * cs^2) -- stated tolerance 1e-3.
-FP32_LINER_TOL = 1e-3
+# to the pulse scale (amp * cs^2) -- stated tolerance 1e-5 (observed ~4e-7 with the
+# bias-free FP32 collision; a rounding bias that accumulates over the run would
+# exceed it).
+FP32_LINER_TOL = 1e-5
 
 
 def test_liner_fp32_close_to_fp64():
