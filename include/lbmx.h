@@ -38,6 +38,7 @@ extern "C" {
 #define LBMX_ERR_HIP (-2)     /* a HIP runtime call failed */
 #define LBMX_ERR_STATE (-3)   /* call not valid in the lattice's current state */
 #define LBMX_ERR_NOMEM (-4)   /* device allocation failed */
+#define LBMX_ERR_DIVERGED (-5) /* a lattice's divergence guard tripped (lbmx_batch.h) */
 
 /* Storage / arithmetic of the populations. */
 #define LBMX_F64 0            /* FP64 populations, FP64 collision (reference arithmetic) */
@@ -232,6 +233,14 @@ LBMX_API int lbmx_body_force_record(lbmx_lattice *lat, double *rows, int capacit
 /* Moments of the current state exactly as compute_macroscopic_fields leaves
  * them after the last step (NaN on solid nodes).  Any pointer may be NULL. */
 LBMX_API int lbmx_moments(lbmx_lattice *lat, double *rho, double *u, double *v);
+
+/* FP64 totals of the moments lbmx_moments would return, over the fluid nodes:
+ * mass = sum rho, energy = sum rho (u^2 + v^2) / 2, in a fixed summation order
+ * (rows dealt to at most 1,024 blocks, a pairwise tree in each, then over the
+ * blocks), so the result depends only on the field -- the full-grid reductions
+ * the reference's drivers form on the host (cases/poiseuille/poiseuille.c:126-137),
+ * here to state how far a reduced-precision run drifts from an FP64 one. */
+LBMX_API int lbmx_totals(lbmx_lattice *lat, double *mass, double *energy);
 
 /* Current populations: f receives what the driver's `f` points to after the
  * last step and swap (the newest state); f_new (may be NULL) what its `f_new`

@@ -138,6 +138,24 @@ LBMX_API int lbmx_batch_moments(lbmx_batch *b, int id, double *rho, double *u, d
 /* Probe history [(steps+1)][nprobe]: row t = lagged rho before step t. Returns rows. */
 LBMX_API int lbmx_batch_probes(lbmx_batch *b, int id, double *hist, int capacity);
 
+/* Divergence guard.  Every collision's moments (the state after the previous
+ * step) are checked at every fluid node; a violation marks the lattice (the word
+ * is the union of the kinds met over its nodes and steps) while the lattices
+ * beside it go on unchanged -- the reference's drivers instead end the whole run
+ * (cases/poiseuille/poiseuille.c:139-142).  A marked lattice's results are not
+ * meaningful; the server stops such a request at the next chunk boundary. */
+#define LBMX_GUARD_NAN 1u     /* a NaN density or velocity */
+#define LBMX_GUARD_RHO 2u     /* density <= 0 (the moments floor it at 1e-16; flagged
+                                 at or below 1e-12) */
+#define LBMX_GUARD_SPEED 4u   /* |u| or |v| above 0.2 (lattice units) */
+/* *guard = the LBMX_GUARD_* bits lattice id has tripped so far (0: healthy). */
+LBMX_API int lbmx_batch_status(lbmx_batch *b, int id, unsigned *guard);
+
+/* FP64 totals of a completed keep_moments lattice's final moments over its fluid
+ * nodes: mass = sum rho, energy = sum rho (u^2 + v^2) / 2, in a fixed summation
+ * order (the result depends only on the field). */
+LBMX_API int lbmx_batch_totals(lbmx_batch *b, int id, double *mass, double *energy);
+
 /* Times candidate launch configurations on the batch as it stands (every lattice
  * must still be waiting for its first step: trial steps run on scratch state and
  * the lattices are initialised afterwards) and keeps the fastest. */
@@ -152,8 +170,10 @@ LBMX_API int lbmx_batch_set_chunk(lbmx_batch *b, int steps);
  * admitted whenever the memory budget can hold them; between graph chunks the
  * finished lattices leave and the waiting ones join, so requests that arrive
  * while others run overlap with them on the device.  A finished request's probe
- * histories are reduced on the device to spectra (lbmx_tl_spec) and its memory
- * is released. */
+ * histories are reduced on the device to spectra (lbmx_tl_spec), its final
+ * moments to FP64 mass and energy, and its memory is released.  A request whose
+ * divergence guard trips is stopped at the next chunk boundary with status
+ * LBMX_ERR_DIVERGED; the requests beside it are not affected. */
 typedef struct {
     int nfreq;                 /* <= 64 frequencies, linear from f_lo to f_hi (cycles/step) */
     double f_lo, f_hi;
@@ -164,12 +184,14 @@ typedef struct {
 
 typedef struct {
     long long tag;             /* as submitted */
-    int status;                /* LBMX_OK or an error code */
+    int status;                /* LBMX_OK, LBMX_ERR_DIVERGED (see guard) or another error */
     int steps;
     double t_submit, t_admit, t_done;   /* seconds on the server's monotonic clock */
     int nfreq;
     double tl_db[64];          /* 20 log10(|P_up(f)| / |P_down(f)|) */
     double up_abs[64], down_abs[64];
+    unsigned guard;            /* LBMX_GUARD_* bits (status LBMX_ERR_DIVERGED) */
+    double mass, energy;       /* FP64 totals of the final state (lbmx_batch_totals) */
 } lbmx_result;
 
 typedef struct lbmx_server lbmx_server;

@@ -20,6 +20,8 @@ from . import native
 from .native import check
 
 WALL_BOTTOM, WALL_TOP, WALL_LEFT, WALL_RIGHT = 1, 2, 4, 8
+GUARD_NAN, GUARD_RHO, GUARD_SPEED = 1, 2, 4
+ERR_DIVERGED = -5
 INIT_PULSE, INIT_TGV, INIT_FIELDS = 0, 1, 2
 CS = 1.0 / math.sqrt(3.0)
 
@@ -63,7 +65,8 @@ class Result(C.Structure):
     _fields_ = [("tag", C.c_longlong), ("status", C.c_int), ("steps", C.c_int),
                 ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
                 ("nfreq", C.c_int), ("tl_db", C.c_double * 64), ("up_abs", C.c_double * 64),
-                ("down_abs", C.c_double * 64)]
+                ("down_abs", C.c_double * 64), ("guard", C.c_uint), ("mass", C.c_double),
+                ("energy", C.c_double)]
 
 
 _bound = False
@@ -106,6 +109,8 @@ def _bind():
     L.lbmx_server_destroy.argtypes = [vp]
     L.lbmx_server_destroy.restype = None
     L.lbmx_batch_spectrum.argtypes = [vp, i, C.POINTER(TlSpec), C.POINTER(Result)]
+    L.lbmx_batch_status.argtypes = [vp, i, C.POINTER(C.c_uint)]
+    L.lbmx_batch_totals.argtypes = [vp, i, C.POINTER(d), C.POINTER(d)]
     _bound = True
     return L
 
@@ -401,6 +406,18 @@ class Batch:
         if n < 0:
             check(n)
         return h[:n, :len(c.probes)]
+
+    def status(self, i):
+        """LBMX_GUARD_* bits lattice i has tripped (0: healthy)."""
+        g = C.c_uint()
+        check(self._L.lbmx_batch_status(self.h, i, C.byref(g)))
+        return g.value
+
+    def totals(self, i):
+        """FP64 (mass, kinetic energy) of completed keep_moments lattice i."""
+        m, e = C.c_double(), C.c_double()
+        check(self._L.lbmx_batch_totals(self.h, i, C.byref(m), C.byref(e)))
+        return m.value, e.value
 
     def spectrum(self, i, tl: TlSpec) -> Result:
         """The server's device spectrum and TL of completed lattice i."""

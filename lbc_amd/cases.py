@@ -188,6 +188,26 @@ def msd_ade(M, Rr, K):
                 alpha=(-1.0, -1.0), beta=(0.0, 0.0))
 
 
+def ade_stable(imp, margin=1e-9):
+    """Whether the impedance ODE decays under the reference's step (nit.c:19-35,
+    dt = 1): a real pole updates phi <- (1 + lambda) phi, so it needs
+    |1 + lambda| < 1; a complex pair updates psi first and chi with the new psi,
+    [[1+a, -b], [b(1+a), 1+a-b^2]], stable when its determinant (1+a)^2 < 1 and
+    |trace| = |2(1+a) - b^2| < 1 + (1+a)^2 (Jury).  An over-damped mass-spring-
+    damper has a fast real pole near -R/M: small masses with large resistances
+    fail the first rule."""
+    for i in range(2):
+        if imp["A"][i] != 0.0 and not abs(1.0 + imp["lambdas"][i]) < 1.0 - margin:
+            return False
+        if imp["B"][i] != 0.0 or imp["C"][i] != 0.0:
+            a, b = imp["alpha"][i], imp["beta"][i]
+            det = (1.0 + a) ** 2
+            tr = 2.0 * (1.0 + a) - b * b
+            if not (det < 1.0 - margin and abs(tr) < 1.0 + det - margin):
+                return False
+    return True
+
+
 @dataclass
 class LinerSpec:
     Nx: int = 8192

@@ -114,6 +114,25 @@ static BatchLayout layout_of(size_t esize, const lbmx_case *c)
     return L;
 }
 
+// Whether the impedance ODE decays under the step of cases/nit/nit.c:19-35 (dt = 1):
+// a real pole updates phi <- (1 + lambda) phi, stable for |1 + lambda| < 1; a
+// complex pair updates psi and then chi with the new psi, amplification
+// [[1+a, -b], [b(1+a), 1+a-b^2]], stable when det = (1+a)^2 < 1 and
+// |trace| = |2(1+a) - b^2| < 1 + det (Jury).  Unused modes (zero weights) are free.
+static bool ade_stable(const lbmx_impedance &z)
+{
+    const double tol = 1e-9;
+    for (int i = 0; i < 2; i++) {
+        if (z.A[i] != 0.0 && !(std::fabs(1.0 + z.lambdas[i]) < 1.0 - tol)) return false;
+        if (z.B[i] != 0.0 || z.C[i] != 0.0) {
+            const double a1 = 1.0 + z.alpha[i], b = z.beta[i];
+            const double det = a1 * a1, tr = 2.0 * a1 - b * b;
+            if (!(det < 1.0 - tol && std::fabs(tr) < 1.0 + det - tol)) return false;
+        }
+    }
+    return true;
+}
+
 static int validate(const lbmx_case *c)
 {
     if (c->Nx < 4 || c->Ny < 4) return lbmx_fail(LBMX_ERR_ARG, "case %dx%d: sizes must be >= 4", c->Nx, c->Ny);
@@ -177,6 +196,9 @@ static int validate(const lbmx_case *c)
             return lbmx_fail(LBMX_ERR_ARG, "impedance opening must lie inside [1, Nx-2]");
         for (int x = c->gap_x0; x <= c->gap_x1; x++)
             if (solid(x, 1)) return lbmx_fail(LBMX_ERR_ARG, "impedance opening under a solid node");
+        if (!ade_stable(c->imp))
+            return lbmx_fail(LBMX_ERR_ARG, "impedance ODE grows under the explicit step "
+                                           "(a real pole outside (-2, 0) or an unstable pair)");
     } else if ((c->walls & LBMX_WALL_BOTTOM) && c->gap_x1 >= c->gap_x0) {
         return lbmx_fail(LBMX_ERR_ARG, "a bottom-wall opening needs the impedance wall");
     }
@@ -225,6 +247,7 @@ static void fill_desc(lbmx_batch *b, BatchSlot &S)
     d.saved[1] = (double *)(blk + L.saved[1]);
     d.state = (double *)(blk + L.state);
     d.probe_hist = (double *)(blk + L.probe);
+    d.status = b->status_dev + (&S - b->slots.data());
     d.start = -1;
     d.plane = L.plane;
     d.Nx = c->Nx;
@@ -498,6 +521,27 @@ const BatchSlot *lbmx_batch_slot(const lbmx_batch *b, int id)
     return &b->slots[id];
 }
 
+int lbmx_batch_totals_of(const lbmx_batch *b, int id, double *scratch, double *out,
+                         hipStream_t stream)
+{
+    const BatchSlot *S = lbmx_batch_slot(b, id);
+    if (!S) return lbmx_fail(LBMX_ERR_ARG, "no lattice %d", id);
+    if (!S->c.keep_moments) return lbmx_fail(LBMX_ERR_STATE, "lattice %d keeps no moments", id);
+    int done = 0;
+    lbmx_batch_steps_done(const_cast<lbmx_batch *>(b), id, &done);
+    if (done < S->c.steps) return lbmx_fail(LBMX_ERR_STATE, "lattice %d has not completed", id);
+    TotalsArgs a{};
+    a.R = S->d.out[0];
+    a.U = S->d.out[1];
+    a.V = S->d.out[2];
+    a.Nx = S->c.Nx;
+    a.Ny = S->c.Ny;
+    a.pitch = S->c.Nx;
+    a.lat = b->lats_dev + id;
+    HIPCHK(launch_totals(a, scratch, out, stream));
+    return LBMX_OK;
+}
+
 int lbmx_tl_check(const lbmx_case *c, const lbmx_tl_spec *tl)
 {
     if (tl->nfreq < 1 || tl->nfreq > 64) return lbmx_fail(LBMX_ERR_ARG, "nfreq must be in [1, 64]");
@@ -587,6 +631,13 @@ LBMX_API int lbmx_batch_create(int precision, size_t budget, lbmx_batch **out)
         rc = lbmx_fail(LBMX_ERR_NOMEM, "tile table");
     if (!rc && hipMalloc((void **)&b->ctl_dev, sizeof(BatchCtl)) != hipSuccess)
         rc = lbmx_fail(LBMX_ERR_NOMEM, "control block");
+    if (!rc && hipMalloc((void **)&b->status_dev, sizeof(unsigned) * kBatchMaxSlots) != hipSuccess)
+        rc = lbmx_fail(LBMX_ERR_NOMEM, "status words");
+    if (!rc && hipMemset(b->status_dev, 0, sizeof(unsigned) * kBatchMaxSlots) != hipSuccess)
+        rc = lbmx_fail(LBMX_ERR_HIP, "status words");
+    if (!rc && hipMalloc((void **)&b->tot_dev, sizeof(double) * (2 * kTotMaxBlocks + 2)) !=
+                   hipSuccess)
+        rc = lbmx_fail(LBMX_ERR_NOMEM, "totals scratch");
     if (!rc) {
         b->ring.resize(64);   // uploads of a burst of admissions rarely wait on a busy slot
         for (auto &s : b->ring)
@@ -621,6 +672,8 @@ LBMX_API void lbmx_batch_destroy(lbmx_batch *b)
     }
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->status_dev) hipFree(b->status_dev);
+    if (b->tot_dev) hipFree(b->tot_dev);
     if (b->spec_dev) hipFree(b->spec_dev);
     if (b->spec_host) hipHostFree(b->spec_host);
     if (b->lats_dev) hipFree(b->lats_dev);
@@ -767,6 +820,33 @@ LBMX_API int lbmx_batch_probes(lbmx_batch *b, int id, double *hist, int capacity
     HIPCHK(hipMemcpy(hist, S->d.probe_hist, (size_t)rows * S->c.nprobe * 8,
                      hipMemcpyDeviceToHost));
     return rows;
+}
+
+LBMX_API int lbmx_batch_status(lbmx_batch *b, int id, unsigned *guard)
+{
+    const BatchSlot *S = lbmx_batch_slot(b, id);
+    if (!S || !guard) return lbmx_fail(LBMX_ERR_ARG, "invalid argument");
+    *guard = 0;
+    if (S->pending) return LBMX_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(guard, b->status_dev + id, sizeof(unsigned), hipMemcpyDeviceToHost));
+    return LBMX_OK;
+}
+
+LBMX_API int lbmx_batch_totals(lbmx_batch *b, int id, double *mass, double *energy)
+{
+    const BatchSlot *S = lbmx_batch_slot(b, id);
+    if (!S || !mass || !energy) return lbmx_fail(LBMX_ERR_ARG, "invalid argument");
+    if (int rc = lbmx_batch_totals_of(b, id, b->tot_dev, b->tot_dev + 2 * kTotMaxBlocks,
+                                      b->stream))
+        return rc;
+    double t[2];
+    HIPCHK(hipMemcpyAsync(t, b->tot_dev + 2 * kTotMaxBlocks, sizeof t, hipMemcpyDeviceToHost,
+                          b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    *mass = t[0];
+    *energy = t[1];
+    return LBMX_OK;
 }
 
 LBMX_API int lbmx_batch_spectrum(lbmx_batch *b, int id, const lbmx_tl_spec *tl,

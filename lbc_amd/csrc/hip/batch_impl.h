@@ -7,6 +7,7 @@
 
 #include "batch_kernels.h"
 #include "lattice_impl.h"
+#include "totals_kernels.h"
 #include "lbmx_batch.h"
 
 constexpr int kBatchMaxSlots = 8192;
@@ -76,6 +77,10 @@ struct lbmx_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // lbmx_batch_spectrum's buffers (allocated on first use)
     double *spec_dev = nullptr, *spec_host = nullptr;
+    // divergence guard word of every descriptor slot (BatchLat::status)
+    unsigned *status_dev = nullptr;
+    // lbmx_batch_totals: partials of the fixed-order reduction, then the two totals
+    double *tot_dev = nullptr;
 };
 
 int lbmx_batch_enqueue(lbmx_batch *b, int nsteps);
@@ -86,3 +91,7 @@ int lbmx_tl_check(const lbmx_case *c, const lbmx_tl_spec *tl);
 // buffer dev and the pinned host buffer host, 4 * 64 doubles each) and the TL.
 int lbmx_batch_spectrum_of(const BatchSlot &S, const lbmx_tl_spec &tl, hipStream_t stream,
                            double *dev, double *host, lbmx_result *out);
+// FP64 mass and kinetic energy of a completed keep_moments lattice into out[0..1]
+// (device), partials in `scratch` (2 * kTotMaxBlocks doubles), on `stream`.
+int lbmx_batch_totals_of(const lbmx_batch *b, int id, double *scratch, double *out,
+                         hipStream_t stream);

@@ -51,6 +51,7 @@ struct BatchLat {
     double *saved[2];        // liner: lagged (rho, u, v) of row 1 over the opening, by parity
     double *state;           // liner: phi[2] psi[2] chi[2] rho_lag (8 doubles) per opening node
     double *probe_hist;      // (steps + 1) x nprobe
+    unsigned *status;        // divergence guard bits (kGuard*), cleared by k_batch_init
     long long start;         // batch epoch at which this lattice takes its step 0
     unsigned plane;
     int Nx, Ny;
@@ -102,6 +103,41 @@ template <typename T>
 __device__ __forceinline__ gptr<const T> gbl(const T *p)
 {
     return (gptr<const T>)p;
+}
+
+// Divergence guard on the moments every collision reads (the state after the
+// previous step).  The reference drivers stop a run once max |u| > 0.2
+// (cases/poiseuille/poiseuille.c:139-142, cases/couette/couette.c:145-148); here
+// a lattice records every kind of violation it meets (the word is the union over
+// its nodes and steps, so it does not depend on scheduling) and the batch keeps
+// running the others, unchanged.
+enum : unsigned { kGuardNaN = 1u, kGuardRho = 2u, kGuardSpeed = 4u };
+constexpr double kGuardSpeedMax = 0.2;
+
+// The density the moments return is floored at 1e-16 (compute_macroscopic_fields.c
+// clamps it there), which the FP32 storage modes rebuild as rho0 + (floor - rho0),
+// i.e. within one rounding of rho0 of the floor (0 or 1.1e-16 for rho0 = 1): so a
+// non-positive population sum shows as a density at or below kGuardRhoMin.
+constexpr double kGuardRhoMin = 1e-12;
+
+__device__ __forceinline__ void guard_check(const BatchLat &L, double rho, double ux, double uy)
+{
+    const bool low = !(rho > kGuardRhoMin);
+    const bool bad = low || !(fabs(ux) <= kGuardSpeedMax) || !(fabs(uy) <= kGuardSpeedMax);
+    if (bad) {
+        const bool nan = rho != rho || ux != ux || uy != uy;
+        const unsigned code = nan ? kGuardNaN : (low ? kGuardRho : kGuardSpeed);
+        // one atomic per wavefront: the union of its tripped lanes' codes
+        const unsigned all = (__ballot(code == kGuardNaN) ? kGuardNaN : 0u) |
+                             (__ballot(code == kGuardRho) ? kGuardRho : 0u) |
+                             (__ballot(code == kGuardSpeed) ? kGuardSpeed : 0u);
+        const unsigned long long lanes = __ballot(1);
+        // skip the atomic once the word holds these bits (a stale read only costs
+        // an atomic more), so a lattice gone bad everywhere does not contend for it
+        if ((unsigned)__lane_id() == (unsigned)__builtin_ctzll(lanes) &&
+            (*gbl((const unsigned *)L.status) & all) != all)
+            __hip_atomic_fetch_or(gbl(L.status), all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 __device__ __forceinline__ bool b_solid(const BatchLat &L, int x, int y)
@@ -236,6 +272,8 @@ __device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, in
         }
         collide_f32(q, m, r0, (float)om, force, (float)L.Fx, (float)L.Fy);
     }
+
+    guard_check(L, rl, ul, vl);
 
     if (F & BF_LINER) {
         if (y == 0 && x >= L.gap_x0 && x <= L.gap_x1) gbl(L.state)[8 * (x - L.gap_x0) + 6] = rl;
@@ -493,6 +531,7 @@ __global__ void __launch_bounds__(256) k_batch_init(const BatchLat *__restrict__
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= L.Nx) return;
     const int n = y * L.Nx + x;
+    if (n == 0) *L.status = 0u;
     double rho, ux, uy;
     if (in.kind == 0) {
         rho = in.prof[x];

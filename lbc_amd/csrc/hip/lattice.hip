@@ -17,6 +17,7 @@
 #include "lbmx.h"
 #include "lattice_impl.h"
 #include "step_kernels.h"
+#include "totals_kernels.h"
 
 using namespace lbmx;
 
@@ -260,6 +261,7 @@ extern "C" void lbmx_destroy(lbmx_lattice *L)
     for (double *p : L->val2) hipFree(p);
     hipFree(L->staging);
     hipFree(L->vtk_dev);
+    hipFree(L->tot_dev);
     if (L->vtk_host) hipHostFree(L->vtk_host);
     hipFree(L->tdibc.st);
     hipFree(L->tdibc.last);
@@ -2023,6 +2025,39 @@ extern "C" int lbmx_write_vtk(lbmx_lattice *L, const char *filename, double dx, 
     }
     if (fclose(fd) != 0 && !rc) rc = lbmx_fail(LBMX_ERR_ARG, "cannot close %s", filename);
     return rc;
+}
+
+extern "C" int lbmx_totals(lbmx_lattice *L, double *mass, double *energy)
+{
+    if (!L || !mass || !energy) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (!L->has_state) return lbmx_fail(LBMX_ERR_STATE, "no state uploaded");
+    if (int prc = lbmx_ensure_push(L)) return prc;
+    TotalsArgs a{};
+    a.R = L->R;
+    a.U = L->U;
+    a.V = L->V;
+    if (L->steps > 0) {
+        if (int rc = lbmx_launch_moments(L)) return rc;
+        a.R = L->Ro;
+        a.U = L->Uo;
+        a.V = L->Vo;
+    }
+    a.Nx = L->Nx;
+    a.Ny = L->Ny;
+    a.pitch = L->pitch;
+    a.cls = L->geom ? L->cls : nullptr;
+    if (!L->tot_dev) {
+        HIPCHK(hipMalloc((void **)&L->tot_dev, sizeof(double) * (2 * kTotMaxBlocks + 2)));
+        L->bytes += sizeof(double) * (2 * kTotMaxBlocks + 2);
+    }
+    double *out = L->tot_dev + 2 * kTotMaxBlocks;
+    HIPCHK(launch_totals(a, L->tot_dev, out, L->stream));
+    double t[2];
+    HIPCHK(hipMemcpyAsync(t, out, sizeof t, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    *mass = t[0];
+    *energy = t[1];
+    return LBMX_OK;
 }
 
 extern "C" int lbmx_synchronize(lbmx_lattice *L)

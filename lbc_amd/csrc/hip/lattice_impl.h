@@ -115,6 +115,8 @@ struct lbmx_lattice {
     // device and a pinned host chunk it is copied through
     uint32_t *vtk_dev = nullptr;
     uint32_t *vtk_host = nullptr;
+    // lbmx_totals: partials of the fixed-order reduction, then the two totals
+    double *tot_dev = nullptr;
 };
 
 lbmx::Geo lbmx_geo(const lbmx_lattice *L);

@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <mutex>
 #include <thread>
@@ -40,11 +41,13 @@ struct Running {
     int id;
     double t_admit;
     long long end_epoch;    // batch epoch after the request's last step
+    unsigned guard = 0;     // divergence guard bits seen at a chunk boundary
 };
 
 struct Inflight {
     hipEvent_t ev;
     long long epoch_after;
+    unsigned *status;       // pinned copy of the guard words after this chunk
 };
 
 }  // namespace
@@ -54,7 +57,9 @@ struct lbmx_server {
     int chunk = 64;
     hipStream_t aux = nullptr;
     double *spec_dev = nullptr;
-    double *spec_host = nullptr;
+    double *spec_host = nullptr;     // spectra (4 x 64 doubles), then the two totals
+    double *tot_dev = nullptr;       // totals partials (2 x kTotMaxBlocks) and result (2)
+    std::vector<unsigned *> status_pool;   // worker only: pinned guard-word copies
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv;
@@ -77,7 +82,16 @@ static int finish(lbmx_server *s, const Running &R, lbmx_result *out)
 {
     const BatchSlot *S = lbmx_batch_slot(s->b, R.id);
     if (!S) return lbmx_fail(LBMX_ERR_STATE, "finished lattice vanished");
-    return lbmx_batch_spectrum_of(*S, R.r.tl, s->aux, s->spec_dev, s->spec_host, out);
+    if (int rc = lbmx_batch_spectrum_of(*S, R.r.tl, s->aux, s->spec_dev, s->spec_host, out))
+        return rc;
+    double *tot = s->tot_dev + 2 * kTotMaxBlocks;
+    if (int rc = lbmx_batch_totals_of(s->b, R.id, s->tot_dev, tot, s->aux)) return rc;
+    HIPCHK(hipMemcpyAsync(s->spec_host + 256, tot, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          s->aux));
+    HIPCHK(hipStreamSynchronize(s->aux));
+    out->mass = s->spec_host[256];
+    out->energy = s->spec_host[257];
+    return LBMX_OK;
 }
 
 static void publish(lbmx_server *s, const Request &r, int status, double t_admit,
@@ -137,7 +151,14 @@ static void worker_main(lbmx_server *s)
             }
             hipEvent_t e = s->ev_pool.back();
             s->ev_pool.pop_back();
+            unsigned *st = s->status_pool.back();
+            s->status_pool.pop_back();
             int rc = lbmx_batch_enqueue(s->b, s->chunk);
+            // the guard words after this chunk, read once the chunk is done
+            if (rc == LBMX_OK &&
+                hipMemcpyAsync(st, s->b->status_dev, sizeof(unsigned) * s->b->slots.size(),
+                               hipMemcpyDeviceToHost, s->b->stream) != hipSuccess)
+                rc = LBMX_ERR_HIP;
             if (rc == LBMX_OK && hipEventRecord(e, s->b->stream) != hipSuccess) rc = LBMX_ERR_HIP;
             if (rc != LBMX_OK) {
                 // fail everything running; the batch is unusable
@@ -150,27 +171,34 @@ static void worker_main(lbmx_server *s)
                 std::lock_guard<std::mutex> g(s->mu);
                 s->n_running = 0;
                 s->ev_pool.push_back(e);
+                s->status_pool.push_back(st);
                 continue;
             }
-            inflight.push_back(Inflight{e, s->b->epoch});
+            inflight.push_back(Inflight{e, s->b->epoch, st});
         }
         // 3. keep at most two chunks in flight; drain when nothing more is queued
         while (!inflight.empty() && (inflight.size() > 1 || !need)) {
             Inflight f = inflight.front();
             inflight.pop_front();
             hipEventSynchronize(f.ev);
+            // only chunks queued after a request's admission carry its guard word (its
+            // slot may have held a diverged lattice before)
+            for (Running &R : s->running)
+                if (f.epoch_after > R.end_epoch - R.r.c.steps) R.guard |= f.status[R.id];
             s->ev_pool.push_back(f.ev);
+            s->status_pool.push_back(f.status);
             done_epoch = f.epoch_after;
         }
-        // 4. retire finished lattices
+        // 4. retire finished lattices and those whose guard tripped
         for (size_t i = 0; i < s->running.size();) {
             const Running &R = s->running[i];
-            if (R.end_epoch > done_epoch) {
+            if (R.end_epoch > done_epoch && !R.guard) {
                 i++;
                 continue;
             }
             lbmx_result res{};
-            const int rc = finish(s, R, &res);
+            res.guard = R.guard;
+            const int rc = R.guard ? LBMX_ERR_DIVERGED : finish(s, R, &res);
             lbmx_batch_remove(s->b, R.id);
             publish(s, R.r, rc, R.t_admit, &res);
             {
@@ -182,6 +210,18 @@ static void worker_main(lbmx_server *s)
     }
     for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
     s->ev_pool.clear();
+}
+
+static int status_pool_init(lbmx_server *s)
+{
+    // one copy per chunk in flight (at most two) and one being queued
+    for (int i = 0; i < 4; i++) {
+        unsigned *p = nullptr;
+        HIPCHK(hipHostMalloc((void **)&p, sizeof(unsigned) * kBatchMaxSlots, hipHostMallocDefault));
+        std::memset(p, 0, sizeof(unsigned) * kBatchMaxSlots);
+        s->status_pool.push_back(p);
+    }
+    return LBMX_OK;
 }
 
 extern "C" {
@@ -202,13 +242,19 @@ LBMX_API int lbmx_server_create(int precision, size_t budget, int chunk,
         rc = lbmx_fail(LBMX_ERR_HIP, "aux stream");
     if (!rc && hipMalloc((void **)&s->spec_dev, 64 * 4 * 8) != hipSuccess)
         rc = lbmx_fail(LBMX_ERR_NOMEM, "spectrum buffer");
-    if (!rc && hipHostMalloc((void **)&s->spec_host, 64 * 4 * 8, hipHostMallocDefault) != hipSuccess)
+    if (!rc && hipHostMalloc((void **)&s->spec_host, (64 * 4 + 2) * 8, hipHostMallocDefault) !=
+                   hipSuccess)
         rc = lbmx_fail(LBMX_ERR_NOMEM, "spectrum staging");
+    if (!rc && hipMalloc((void **)&s->tot_dev, (2 * kTotMaxBlocks + 2) * 8) != hipSuccess)
+        rc = lbmx_fail(LBMX_ERR_NOMEM, "totals scratch");
+    if (!rc) rc = status_pool_init(s);
     if (rc) {
         if (s->b) lbmx_batch_destroy(s->b);
         if (s->aux) hipStreamDestroy(s->aux);
         if (s->spec_dev) hipFree(s->spec_dev);
         if (s->spec_host) hipHostFree(s->spec_host);
+        if (s->tot_dev) hipFree(s->tot_dev);
+        for (unsigned *p : s->status_pool) hipHostFree(p);
         delete s;
         return rc;
     }
@@ -225,6 +271,7 @@ LBMX_API int lbmx_server_submit(lbmx_server *s, const lbmx_case *c, const lbmx_t
     if (c->init == LBMX_INIT_FIELDS)
         return lbmx_fail(LBMX_ERR_ARG, "served requests start from a generated state");
     Request r{*c, *tl, tag, s->now()};
+    r.c.keep_moments = 1;   // the final moments give the result's mass and energy
     {
         std::lock_guard<std::mutex> g(s->mu);
         if (s->stop) return lbmx_fail(LBMX_ERR_STATE, "server is stopping");
@@ -273,6 +320,8 @@ LBMX_API void lbmx_server_destroy(lbmx_server *s)
     if (s->aux) hipStreamDestroy(s->aux);
     if (s->spec_dev) hipFree(s->spec_dev);
     if (s->spec_host) hipHostFree(s->spec_host);
+    if (s->tot_dev) hipFree(s->tot_dev);
+    for (unsigned *p : s->status_pool) hipHostFree(p);
     delete s;
 }
 

@@ -98,6 +98,7 @@ def _bind():
     L.lbmx_mei_initialization.argtypes = [i, i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                           i, i, i, d, _ip]
     L.lbmx_forces.argtypes = [vp, _dp, _dp]
+    L.lbmx_totals.argtypes = [vp, _dp, _dp]
     L.lbmx_set_force_record.argtypes = [vp, d, d, i]
     L.lbmx_force_record.argtypes = [vp, _dp, i]
     L.lbmx_set_force_body.argtypes = [vp, _ip]
@@ -320,6 +321,13 @@ class Lattice:
         Lf, Df = C.c_double(), C.c_double()
         check(self._L.lbmx_forces(self.h, C.byref(Lf), C.byref(Df)))
         return Lf.value, Df.value
+
+    def totals(self):
+        """FP64 (mass, kinetic energy) of the current moments over the fluid nodes,
+        fixed summation order (lbmx_totals)."""
+        m, e = C.c_double(), C.c_double()
+        check(self._L.lbmx_totals(self.h, C.byref(m), C.byref(e)))
+        return m.value, e.value
 
     def set_force_body(self, body):
         """Solid nodes (nonzero in the Ny x Nx array) whose links form the body

@@ -30,7 +30,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import batch as B
-from .cases import msd_ade
+from .cases import ade_stable, msd_ade
 
 CS = B.CS
 NFREQ = 32
@@ -99,10 +99,12 @@ def draw_requests(n, seed=7, Nx=(1024, 8192), Ny=(64, 256), R=(0.5, 3.0),
     """n seeded requests: Nx, Ny uniform (multiples of 32); R, f_res, mass and
     amplitude uniform in the given ranges; the length uniform in the given range
     above min_steps(Nx) (a shorter run has no transmitted pulse to measure, so
-    such a draw is drawn again).  Every draw is stable under the reference's ODE
-    step: it updates chi with the new psi (nit.c:31), which makes the pair's
-    amplification matrix [[1+a, -b], [b(1+a), 1+a-b^2]] with determinant
-    (1+a)^2 < 1 for the damped poles a +- ib."""
+    such a draw is drawn again).  The mass is drawn again until the impedance ODE
+    is stable under the reference's explicit step (cases.ade_stable): an
+    over-damped draw with a small mass has a real pole below -2, which that step
+    amplifies (|1 + lambda| > 1) until the request diverges; a complex pair is
+    updated in place (nit.c:31), amplification [[1+a, -b], [b(1+a), 1+a-b^2]] with
+    determinant (1+a)^2 < 1 for the damped poles a +- ib."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
@@ -111,6 +113,8 @@ def draw_requests(n, seed=7, Nx=(1024, 8192), Ny=(64, 256), R=(0.5, 3.0),
         r = float(rng.uniform(*R))
         fr = float(rng.uniform(*f_res))
         m = float(rng.uniform(*M))
+        while not ade_stable(LinerRequest(nx, ny, r, fr, m, 0.0, 0).impedance()):
+            m = float(rng.uniform(*M))
         a = float(rng.uniform(*amp))
         if min_steps(nx) > steps[1]:
             raise ValueError(f"steps up to {steps[1]} cannot cover a {nx}-wide duct")

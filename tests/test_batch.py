@@ -136,6 +136,12 @@ def test_case_check_rules():
     c = B.couette(64, 64, 0.8, 10)
     c.convective_ends = True                # with a moving lid
     bad.append(c)
+    # over-damped impedance whose fast real pole (-2.18) the explicit step amplifies
+    from lbc_amd import serving as SV
+    fast = SV.LinerRequest(640, 64, 2.2405399916753885, 0.02347702808796827,
+                           1.0223512526325425, 1e-3, 3649)
+    assert not cases.ade_stable(fast.impedance())
+    bad.append(B.liner_duct(1024, 64, 10, fast.impedance()))
     for c in bad:
         with pytest.raises(native.LbmxError):
             B.case_check(c)

@@ -44,8 +44,9 @@ def test_request_distribution():
 
 
 def euler_growth(imp, steps=4000):
-    """Largest |state| after driving the explicit-Euler ODE (nit.c:13-47) with a
-    unit impulse of pressure and then nothing."""
+    """Largest |state| of every mode the impedance uses, over the last 100 steps,
+    after driving the explicit-Euler ODE (nit.c:13-47) with a unit impulse of
+    pressure and then nothing."""
     phi, psi, chi = [0.0, 0.0], [0.0, 0.0], [0.0, 0.0]
     peak = 0.0
     for t in range(steps):
@@ -55,14 +56,42 @@ def euler_growth(imp, steps=4000):
             psi[i] = psi[i] + (p + imp["alpha"][i] * psi[i] - imp["beta"][i] * chi[i])
             chi[i] = chi[i] + (imp["beta"][i] * psi[i] + imp["alpha"][i] * chi[i])
         if t > steps - 100:
-            peak = max(peak, abs(psi[0]) + abs(chi[0]))
+            for i in range(2):
+                mags = []
+                if imp["A"][i] != 0.0:
+                    mags.append(abs(phi[i]))
+                if imp["B"][i] != 0.0 or imp["C"][i] != 0.0:
+                    mags.append(abs(psi[i]) + abs(chi[i]))
+                for m in mags:
+                    if not math.isfinite(m):     # overflowed (inf, or inf - inf = NaN)
+                        return math.inf
+                    peak = max(peak, m)
     return peak
 
 
 def test_every_request_ode_is_stable():
-    for r in SV.draw_requests(200, seed=4):
+    rs = SV.draw_requests(400, seed=4)
+    assert any(r.impedance()["A"][0] != 0.0 for r in rs)      # over-damped draws occur
+    for r in rs:
         imp = r.impedance()
         assert euler_growth(imp) < 1.0, r
+
+
+def test_stability_rule_matches_the_ode():
+    """cases.ade_stable agrees with the ODE itself on mass-spring-dampers across
+    the drawn ranges, including the over-damped ones with a fast real pole
+    below -2 (R = 2.24, M = 1.02 rho0 c0: lambda = -2.18)."""
+    from lbc_amd.cases import ade_stable
+    bad = SV.LinerRequest(640, 64, 2.2405399916753885, 0.02347702808796827, 1.0223512526325425,
+                          1e-3, 3649)
+    assert not ade_stable(bad.impedance()) and euler_growth(bad.impedance()) > 1e6
+    rng = np.random.default_rng(8)
+    for _ in range(300):
+        r = SV.LinerRequest(1024, 64, rng.uniform(0.5, 3.0), rng.uniform(1 / 80, 1 / 20),
+                            rng.uniform(1.0, 16.0), 1e-3, 2000)
+        imp = r.impedance()
+        g = euler_growth(imp, steps=6000)
+        assert ade_stable(imp) == (g < 1.0), (r, g)
 
 
 def test_in_place_step_amplification():

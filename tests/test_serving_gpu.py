@@ -61,7 +61,8 @@ def test_same_result_alone_and_in_mix():
         s.submit(rs[i].case(), rs[i].tl_spec(), i)
     mix = wait_all(s, 6)
     s.close()
-    assert alone.status == 0 and all(r.status == 0 for r in mix.values())
+    assert alone.status == 0 and all(r.status == 0 for r in mix.values()), \
+        [(t, r.status, r.guard) for t, r in mix.items()]
     for a, b in zip(as_arrays(alone), as_arrays(mix[101])):
         assert np.array_equal(a, b)
     # x really shared the device: its residency overlapped other requests'
@@ -130,7 +131,7 @@ def test_rigid_duct_and_liner_spectra():
 
 def test_budget_admission_and_completion():
     r = requests(1, seed=9)[0]
-    one = B.case_bytes(r.case(), PREC)
+    one = B.case_bytes(r.case(keep_moments=True), PREC)   # served cases keep their moments
     budget = int(2.2 * one)             # room for two requests at a time
     s = B.Server(PREC, budget=budget, chunk=32)
     for i in range(5):
@@ -150,7 +151,7 @@ def test_budget_admission_and_completion():
 def test_request_that_cannot_fit_fails_cleanly():
     big = SV.LinerRequest(8192, 256, 1.0, 1 / 40, 4.0, 5e-4, SV.min_steps(8192))
     small = requests(1)[0]
-    s = B.Server(PREC, budget=int(1.5 * B.case_bytes(small.case(), PREC)))
+    s = B.Server(PREC, budget=int(1.5 * B.case_bytes(small.case(keep_moments=True), PREC)))
     s.submit(big.case(), big.tl_spec(), 0)
     s.submit(small.case(), small.tl_spec(), 1)
     got = wait_all(s, 2)
