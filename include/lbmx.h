@@ -242,6 +242,23 @@ LBMX_API int lbmx_moments(lbmx_lattice *lat, double *rho, double *u, double *v);
  * here to state how far a reduced-precision run drifts from an FP64 one. */
 LBMX_API int lbmx_totals(lbmx_lattice *lat, double *mass, double *energy);
 
+/* No counterpart in the reference.  lbmx_totals for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops): the same
+ * kernels and summation order over caller-owned device arrays, so the totals of
+ * a field equal lbmx_totals' bit for bit.  rho, u, v: device arrays [Ny][Nx]
+ * (row-major, the reference's layout); mask: device int [Ny][Nx], nonzero =
+ * solid (skipped), or NULL for an all-fluid lattice; out: device memory for two
+ * doubles {mass, energy}.  Enqueued on `stream` (a hipStream_t; NULL = the null
+ * stream); only kernel launches, no allocation, no copy, no synchronization.
+ * Its scratch is a fixed LBMX_NOALLOC_SCRATCH_BYTES device array per GPU, held
+ * until the process ends; it covers any lattice of 1 .. LBMX_TOTALS_NOALLOC_MAX_NODES
+ * nodes (other sizes: LBMX_ERR_ARG).  Calls that overlap on one GPU (several
+ * streams, or threads) share that array and race. */
+#define LBMX_NOALLOC_SCRATCH_BYTES 16384
+#define LBMX_TOTALS_NOALLOC_MAX_NODES 2147483647LL
+LBMX_API int lbmx_totals_noalloc(int Nx, int Ny, const double *rho, const double *u,
+                                 const double *v, const int *mask, double *out, void *stream);
+
 /* Current populations: f receives what the driver's `f` points to after the
  * last step and swap (the newest state); f_new (may be NULL) what its `f_new`
  * points to (the post-collision field of the last step, stale entries included).
