@@ -171,24 +171,38 @@ def fp32_accuracy(N=4096, steps=2000):
         lat = tgv_lattice(N, prec)
         lat.step(steps)
         _, u, v = lat.moments()
+        tot = lat.totals()
         lat.close()
-        return u, v
+        return u, v, tot
 
-    u64, v64 = vel(native.F64)
+    u64, v64, t64 = vel(native.F64)
     den = float((u64 ** 2 + v64 ** 2).sum())
     out = {"workload": f"taylor_green_periodic_{N}x{N}_{steps}_steps"}
-    for prec, key in ((native.F32_SHIFTED, "fp32_collision_vel_l2_vs_fp64"),
-                      (native.F32_WIDE, "fp64_collision_vel_l2_vs_fp64")):
-        u, v = vel(prec)
-        out[key] = float(f"{math.sqrt(float(((u - u64) ** 2 + (v - v64) ** 2).sum()) / den):.3e}")
+    for prec, key in ((native.F32_SHIFTED, "fp32_collision"), (native.F32_WIDE, "fp64_collision")):
+        u, v, t = vel(prec)
+        out[key + "_vel_l2_vs_fp64"] = float(
+            f"{math.sqrt(float(((u - u64) ** 2 + (v - v64) ** 2).sum()) / den):.3e}")
+        out[key + "_mass_rel_diff"] = float(f"{abs(t[0] - t64[0]) / t64[0]:.3e}")
+        out[key + "_energy_rel_diff"] = float(f"{abs(t[1] - t64[1]) / t64[1]:.3e}")
     return out
 
 
-def cylinder_lattice(precision):
-    """Channel past a cylinder, 2048 x 512, Re = 100 (configs' channel case)."""
+CYL = dict(Nx=2048, Ny=512, R=32.0, xc=512.0, yc=256.0, umax=0.05, Re=100.0)
+
+
+def cylinder_lattice(precision, start="inflow", kick=0.0):
+    """Channel past a cylinder, 2048 x 512, Re = 100 on u_max and D (the channel
+    configuration).  start="inflow": every fluid node starts at the inlet's
+    parabolic profile, as the reference's cylinder driver starts from its inflow
+    velocity (cases/cylinder/cylinder.c:56); start="rest": u = 0, whose sudden
+    inflow launches a density step of u_max / cs ~ 9 % that rings between the
+    reflecting inlet and outlet.  kick: a transverse velocity kick * u_max *
+    exp(-r^2 / R^2) centred 3 R behind the cylinder, which breaks the wake's
+    symmetry so that shedding starts without waiting for round-off to grow."""
+    from lbc_amd import cases as CS_
     from lbc_amd import native
-    Nx, Ny, R, xc, yc, umax = 2048, 512, 32.0, 512.0, 256.0, 0.05
-    nu = umax * 64 / 100.0
+    Nx, Ny, R, xc, yc, umax = (CYL[k] for k in ("Nx", "Ny", "R", "xc", "yc", "umax"))
+    nu = umax * 2 * R / CYL["Re"]
     tau = 3.0 * nu + 0.5
     j, i = np.mgrid[0:Ny, 0:Nx]
     mask = ((j == 0) | (j == Ny - 1)).astype(np.int32)
@@ -207,9 +221,37 @@ def cylinder_lattice(precision):
     lat.set_bc_values(1, np.ones(Ny), None)
     rho = np.ones((Ny, Nx))
     zero = np.zeros((Ny, Nx))
-    f = np.broadcast_to(np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4), (Ny, Nx, 9)).copy()
-    lat.upload(f, f, rho, zero, zero)
+    u = np.where(mask == 0, uin[:, None], 0.0) if start == "inflow" else zero
+    v = zero
+    if kick:
+        v = np.where(mask == 0, kick * umax * np.exp(-((i - xc - 3 * R) ** 2 + (j - yc) ** 2)
+                                                       / (R * R)), 0.0)
+    f = CS_.equilibrium(rho, u, v)
+    lat.upload(f, f, rho, u, v)
     return lat
+
+
+def force_stats(body, window_conv=20.0):
+    """Force coefficients of the cylinder alone over the last `window_conv`
+    convective times D / u_max (cases/cylinder/process.py:105-150: C = F /
+    (rho U^2 R), mean and standard deviation, Strouhal number of the lift
+    spectrum's peak), with U = u_max; the same with the mean inflow 2/3 u_max is
+    given for the drag.  The lift peak is located on an 8x zero-padded spectrum."""
+    D, U = 2 * CYL["R"], CYL["umax"]
+    T = D / U
+    n = len(body)
+    w = int(min(n, round(window_conv * T)))
+    L, Dr = body[n - w:, 1], body[n - w:, 2]
+    q = 0.5 * 1.0 * U * U * D
+    CL, CD = L / q, Dr / q
+    spec = np.abs(np.fft.rfft(CL - CL.mean(), n=8 * w))
+    freq = np.fft.rfftfreq(8 * w, 1.0)
+    k = int(np.argmax(spec[1:]) + 1)
+    return {"window_steps": w, "window_convective_times": round(w / T, 1),
+            "cd_mean": round(float(CD.mean()), 4), "cd_std": round(float(CD.std()), 4),
+            "cd_mean_on_mean_inflow": round(float(CD.mean() * 9.0 / 4.0), 4),
+            "cl_mean": round(float(CL.mean()), 4), "cl_rms": round(float(CL.std()), 4),
+            "strouhal": round(float(freq[k] * D / U), 4)}
 
 
 def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True, precision=None):
@@ -296,15 +338,17 @@ def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True, precision
     }
 
 
-def serving_bench(rates=(10, 30, 100, 300, 1000), n=32, seq_sample=6, seed=2026,
-                  budget_gib=200):
+def serving_bench(rates=(10, 30, 100, 300, 1000), n=128, seed=2026, budget_gib=200):
     """The serving loop: n seeded liner-duct requests (Nx in [1024, 8192], Ny in
     [64, 256], 2,000-20,000 steps, seeded impedance) arriving as a Poisson stream
     at each rate, admitted by a continuous-batching server (FP32 storage, FP64
-    collision) under an HBM budget; each returns a TL spectrum.  Throughput =
-    requests / (last completion - first arrival).  Baseline: a seeded sample of the
-    same requests solved one after another with plain single-lattice FP64 launches
-    (stepping time only), which also gives the served spectra's TL error."""
+    collision) under an HBM budget; each returns a TL spectrum and the FP64 mass
+    and energy of its final state.  Throughput = requests / (last completion -
+    first arrival); latency = completion - submission, split into the queue wait
+    (submission -> admission) and the device residency (admission -> completion).
+    Baseline: every one of the same n requests solved one after another with plain
+    single-lattice FP64 launches (stepping time only), which also gives each served
+    result's TL, mass and energy error."""
     from lbc_amd import batch as B
     from lbc_amd import native
     from lbc_amd import serving as SV
@@ -323,6 +367,13 @@ def serving_bench(rates=(10, 30, 100, 300, 1000), n=32, seq_sample=6, seed=2026,
         log(f"server with a {budget_gib} GiB budget failed ({e}); using 64 GiB")
         server = B.Server(prec, budget=64 << 30, chunk=64, tuning=tuning)
         budget = 64
+
+    def pct(a):
+        return {"p50_s": round(float(np.percentile(a, 50)), 4),
+                "p95_s": round(float(np.percentile(a, 95)), 4),
+                "p99_s": round(float(np.percentile(a, 99)), 4),
+                "mean_s": round(float(np.mean(a)), 4)}
+
     sweep = []
     served = {}
     for rate in rates:
@@ -331,23 +382,23 @@ def serving_bench(rates=(10, 30, 100, 300, 1000), n=32, seq_sample=6, seed=2026,
         bad = [r.status for r in res.values() if r.status != 0]
         t0 = min(r.t_submit for r in res.values())
         t1 = max(r.t_done for r in res.values())
-        lat = SV.latency_summary(res)
+        lat = np.array([r.t_done - r.t_submit for r in res.values()])
+        wait = np.array([r.t_admit - r.t_submit for r in res.values()])
+        resid = np.array([r.t_done - r.t_admit for r in res.values()])
         row = {"rate_per_s": rate, "requests": n, "solves_per_s": round(n / (t1 - t0), 3),
-               "p50_s": round(lat["p50_s"], 4), "p95_s": round(lat["p95_s"], 4),
-               "p99_s": round(lat["p99_s"], 4), "mean_s": round(lat["mean_s"], 4),
+               **pct(lat), "queue_wait": pct(wait), "device_residency": pct(resid),
                "failed": len(bad)}
         sweep.append(row)
         served = res
-        log(f"serving rate {rate}/s: {row}")
+        log(f"serving rate {rate}/s: {n / (t1 - t0):.2f} solves/s, latency p50 "
+            f"{row['p50_s']:.2f} p99 {row['p99_s']:.2f} s (queue p50 "
+            f"{row['queue_wait']['p50_s']:.2f} s, residency p50 "
+            f"{row['device_residency']['p50_s']:.2f} s), failed {len(bad)}")
     peak = server.stats()["peak_bytes"]
     server.close()
 
-    rng = np.random.default_rng(seed + 1)
-    pick = sorted(rng.choice(n, size=min(seq_sample, n), replace=False).tolist())
-    seq_t, tl_err = [], []
-    f = np.linspace(SV.F_LO, SV.F_HI, SV.NFREQ)
-    for i in pick:
-        r = reqs[i]
+    seq_t, tl_err, mass_err, energy_err = [], [], [], []
+    for i, r in enumerate(reqs):
         lat = r.case().single_lattice(native.F64, record_steps=r.steps + 1)
         lat.synchronize()
         t1 = time.perf_counter()
@@ -355,26 +406,83 @@ def serving_bench(rates=(10, 30, 100, 300, 1000), n=32, seq_sample=6, seed=2026,
         lat.synchronize()
         seq_t.append(time.perf_counter() - t1)
         hist = lat.probe_history()
+        m64, e64 = lat.totals()
         lat.close()
         tl64, up64, _ = SV.tl_from_history(hist, r)
-        tl32 = np.array(served[i].tl_db[:SV.NFREQ])
+        s32 = served[i]
+        tl32 = np.array(s32.tl_db[:SV.NFREQ])
         band = up64 > 1e-2 * up64.max()
         tl_err.append(float(np.abs(tl32 - tl64)[band].max()))
-    seq_rate = 1.0 / float(np.mean(seq_t))
+        mass_err.append(abs(s32.mass - m64) / m64)
+        energy_err.append(abs(s32.energy - e64) / e64)
+        if (i + 1) % 16 == 0:
+            log(f"sequential FP64: {i + 1} of {n} requests, {sum(seq_t):.1f} s")
+    seq_total = float(np.sum(seq_t))
+    seq_rate = n / seq_total
     sat = max(row["solves_per_s"] for row in sweep)
-    log(f"sequential FP64 sample {len(pick)}: {np.mean(seq_t):.3f} s per solve "
-        f"({seq_rate:.2f} solves/s); served max {sat:.2f} solves/s ({sat / seq_rate:.2f}x); "
-        f"TL difference vs FP64 max {max(tl_err):.2e} dB")
+    log(f"sequential FP64, all {n} requests: {seq_total:.1f} s ({seq_rate:.2f} solves/s); served "
+        f"max {sat:.2f} solves/s ({sat / seq_rate:.2f}x); TL difference vs FP64 max "
+        f"{max(tl_err):.2e} dB; mass {max(mass_err):.1e}, energy {max(energy_err):.1e} relative")
     return {
         "workload": f"serving_liner_requests_{n}_per_rate_fp32_storage_fp64_collision",
         "budget_GiB": budget, "peak_bytes": int(peak), "chunk_steps": 64,
         "tuning": {"threads": tuning[0], "tile": tuning[1], "groups_per_cu": tuning[2],
                    "order": tuning[3]},
         "sweep": sweep,
-        "sequential_fp64_solves_per_s": round(seq_rate, 3), "sequential_sample": len(pick),
+        "sequential_fp64_solves_per_s": round(seq_rate, 3), "sequential_sample": n,
         "speedup_vs_sequential_at_saturation": round(sat / seq_rate, 2),
         "tl_max_abs_diff_db_vs_fp64": float(f"{max(tl_err):.3e}"),
+        "mass_max_rel_diff_vs_fp64": float(f"{max(mass_err):.3e}"),
+        "energy_max_rel_diff_vs_fp64": float(f"{max(energy_err):.3e}"),
     }
+
+
+def dropin_bench(steps=100):
+    """The drop-in boundary as the reference's drivers use it: main_lbm called once
+    per step on host arrays with the driver's swap (cases/cylinder/cylinder.c:170-190),
+    on the channel past a cylinder 2048 x 512.  The library keeps the lattice on the
+    device between calls but must return rho, u, v (and the forces) to the caller
+    every step, so the rate is set by the PCIe copies, not by the kernels."""
+    from lbc_amd import capi, cases
+    Nx, Ny, R, xc, yc, umax = 2048, 512, 32.0, 512.0, 256.0, 0.05
+    nu = umax * 2 * R / 100.0
+    tau = 3.0 * nu + 0.5
+    j, i = np.mgrid[0:Ny, 0:Nx]
+    walls = (j == 0) | (j == Ny - 1)
+    d = np.sqrt((i - xc) ** 2 + (j - yc) ** 2) - R
+    mask = (walls | (d <= 0)).astype(np.int32)
+    phi = np.where(walls, -1e30, d)
+    omega = np.full((Ny, Nx), 1.0 / tau)
+    y = np.arange(Ny) - 0.5
+    H = Ny - 2.0
+    uin = np.clip(4 * umax * y * (H - y) / H ** 2, 0.0, None)
+    bcs = [("velocity_inlet_regularized", 0, uin, np.zeros(Ny)),
+           ("pressure_outlet_regularized", Nx - 1, np.ones(Ny), None)]
+    rho = np.ones((Ny, Nx))
+    u = np.zeros((Ny, Nx))
+    v = np.zeros((Ny, Nx))
+    f = cases.equilibrium(rho, u, v)
+    f_new = f.copy()
+    D = capi.DropIn()
+    tbl = D.table(bcs)
+
+    def run(n):
+        nonlocal f, f_new
+        for _ in range(n):
+            D.main_lbm(f, f_new, rho, u, v, omega, mask, bcs, 0, 0, 0.0, 0.0, phi, 1, table=tbl)
+            f, f_new = f_new, f
+    run(5)
+    t0 = time.perf_counter()
+    run(steps)
+    wall = time.perf_counter() - t0
+    D.release()
+    out = {"workload": f"dropin_main_lbm_cylinder_2048x512_fp64_{steps}_steps",
+           "ms_per_step": round(wall / steps * 1e3, 3),
+           "MLUPS": round(Nx * Ny * steps / wall / 1e6, 1),
+           "host_bytes_per_step": 3 * 8 * Nx * Ny,
+           "note": "rho, u, v returned to host arrays every call (the drivers read them)"}
+    log(f"drop-in main_lbm: {out}")
+    return out
 
 
 def vtk_output_times(lat, reps=3):
@@ -402,16 +510,24 @@ def vtk_output_times(lat, reps=3):
 
 def cylinder_bench(steps=50000):
     """Channel past a cylinder, 2048 x 512, Re = 100: FP64 and FP32 storage with the
-    FP64 collision, `steps` steps each with the drag/lift history recorded on the
-    device every step (the cylinder driver's {t, L, D} record, walls included, and
-    the cylinder alone beside it).  Reported: MLUPS, the FP32 history's largest
-    deviation from FP64, relative to the FP64 run's largest |D| and |L|, the last
-    step's forces, and one VTK output step packed on the device vs on the host."""
+    FP64 collision, `steps` steps each from the inflow profile with the drag/lift
+    history recorded on the device every step (the cylinder driver's {t, L, D}
+    record, walls included, and the cylinder alone beside it), and FP64 once more
+    from rest.  Reported: MLUPS, the force statistics of the cylinder alone over
+    the last 20 convective times (mean C_D, C_L rms, St; references of
+    cases/cylinder/process.py: the C_D correlation of unbounded uniform flow, 1.45
+    at Re = 100 (:63), St 0.21 (:225), C_L std 0.4), the FP32 history's largest
+    deviation from FP64, relative to the FP64 run's largest |D| and |L|, the FP64
+    totals, and one VTK output step packed on the device vs on the host."""
     from lbc_amd import native
-    out = {"workload": f"cylinder_channel_2048x512_re100_ibb_regularized_{steps}_steps"}
+    out = {"workload": f"cylinder_channel_2048x512_re100_ibb_regularized_{steps}_steps",
+           "reference_values": {"cd_mean_correlation_unbounded": 1.449, "strouhal": 0.21,
+                                "cl_std": 0.4}}
     rec = {}
-    for name, prec in (("fp64", native.F64), ("fp32_fp64collision", native.F32_WIDE)):
-        lat = cylinder_lattice(prec)
+    runs = (("fp64", native.F64, "inflow"), ("fp32_fp64collision", native.F32_WIDE, "inflow"),
+            ("fp64_from_rest", native.F64, "rest"))
+    for name, prec, start in runs:
+        lat = cylinder_lattice(prec, start)
         lat.set_force_record(steps)
         lat.synchronize()
         t0 = time.perf_counter()
@@ -419,30 +535,43 @@ def cylinder_bench(steps=50000):
         lat.synchronize()
         wall = time.perf_counter() - t0
         rec[name] = lat.force_record()
+        body = lat.body_force_record()
+        stats = force_stats(body)
+        mass, energy = lat.totals()
         if name == "fp64":
-            body = lat.body_force_record()
-            out["cylinder_only_last"] = {"drag": float(body[-1, 2]), "lift": float(body[-1, 1])}
             out["vtk_output"] = vtk_output_times(lat)
         lat.close()
         out[name] = {"MLUPS": round(2048 * 512 * steps / wall / 1e6, 1),
-                     "ms_per_step": round(wall / steps * 1e3, 4)}
+                     "ms_per_step": round(wall / steps * 1e3, 4), "forces": stats,
+                     "mass": mass, "kinetic_energy": energy}
+        log(f"cylinder {name}: {out[name]}")
     r64, r32 = rec["fp64"], rec["fp32_fp64collision"]
     n = min(len(r64), len(r32))
-    out["fp32_fp64collision"]["drag_err"] = float(
-        f"{np.abs(r32[:n, 2] - r64[:n, 2]).max() / np.abs(r64[:n, 2]).max():.3e}")
-    out["fp32_fp64collision"]["lift_err"] = float(
-        f"{np.abs(r32[:n, 1] - r64[:n, 1]).max() / np.abs(r64[:n, 1]).max():.3e}")
-    out["fp64"]["drag_last"] = float(r64[n - 1, 2])
-    out["fp64"]["lift_last"] = float(r64[n - 1, 1])
-    out["vs_fp64"] = round(out["fp32_fp64collision"]["MLUPS"] / out["fp64"]["MLUPS"], 3)
-    log(f"cylinder: {out}")
+    o32 = out["fp32_fp64collision"]
+    o32["drag_err"] = float(f"{np.abs(r32[:n, 2] - r64[:n, 2]).max() / np.abs(r64[:n, 2]).max():.3e}")
+    o32["lift_err"] = float(f"{np.abs(r32[:n, 1] - r64[:n, 1]).max() / np.abs(r64[:n, 1]).max():.3e}")
+    o32["mass_rel_diff"] = float(f"{abs(o32['mass'] - out['fp64']['mass']) / out['fp64']['mass']:.3e}")
+    o32["energy_rel_diff"] = float(
+        f"{abs(o32['kinetic_energy'] - out['fp64']['kinetic_energy']) / out['fp64']['kinetic_energy']:.3e}")
+    out["vs_fp64"] = round(o32["MLUPS"] / out["fp64"]["MLUPS"], 3)
+    # shedding needs longer than the configuration's run: FP64 for 200,000 steps
+    # (156 convective times) with a wake kick, statistics over the last 20
+    long_steps = 200000
+    lat = cylinder_lattice(native.F64, "inflow", kick=0.3)
+    lat.set_force_record(long_steps)
+    lat.step(long_steps)
+    out["fp64_shedding_200k_steps"] = {"kick": 0.3, "forces": force_stats(lat.body_force_record())}
+    out["fp64_shedding_200k_steps"]["forces"]["strouhal_on_mean_inflow"] = round(
+        out["fp64_shedding_200k_steps"]["forces"]["strouhal"] * 1.5, 4)
+    lat.close()
+    log(f"cylinder shedding run: {out['fp64_shedding_200k_steps']}")
     return out
 
 
-def liner_bench(steps=20000):
+def liner_bench(steps=20000, seed=2025):
     """The acoustic liner duct, 8192 x 256, tau = 0.51: Gaussian pulse (1e-3 rho0,
     half-width 20, x = 1024) at rest, rigid top wall, impedance wall (mass-spring-
-    damper, R = rho0 c0, resonance 1/40 per step, M = 8) on x in [3072, 5120] of the
+    damper, R = rho0 c0, resonance 1/40 per step, seeded mass M) on x in [3072, 5120] of the
     bottom wall, convective ends with sponges; 20,000 steps in FP64 (bit-identical
     to the oracle) and in both FP32-storage modes.  Reported: time per run, and the
     FP32 modes' largest error, relative to the pulse's p' amplitude, in the wall
@@ -451,9 +580,17 @@ def liner_bench(steps=20000):
     from lbc_amd import batch as B
     from lbc_amd import cases as CS_
     from lbc_amd import native
-    imp = CS_.LinerSpec(R=1.0, f_res=1.0 / 40.0, M=8.0).impedance()
+    # seeded reactance slope: the mass drawn uniform in [1, 16] rho0 c0 steps (the
+    # serving requests' range) until the ODE is stable under the explicit step
+    rng = np.random.default_rng(seed)
+    M = float(rng.uniform(1.0, 16.0))
+    while not CS_.ade_stable(CS_.LinerSpec(R=1.0, f_res=1.0 / 40.0, M=M).impedance()):
+        M = float(rng.uniform(1.0, 16.0))
+    imp = CS_.LinerSpec(R=1.0, f_res=1.0 / 40.0, M=M).impedance()
     probes = [(3584, 0), (4096, 0), (4608, 0), (6144, 128)]
-    out = {"workload": f"liner_duct_8192x256_tau0.51_{steps}_steps"}
+    out = {"workload": f"liner_duct_8192x256_tau0.51_{steps}_steps",
+           "impedance": {"R_over_rho0c0": 1.0, "f_res_per_step": 1.0 / 40.0,
+                         "M_over_rho0c0": round(M, 4), "seed": seed}}
     runs = {}
     for name, prec in (("fp64", native.F64), ("fp32_fp32collision", native.F32_SHIFTED),
                        ("fp32_fp64collision", native.F32_WIDE)):
@@ -465,9 +602,11 @@ def liner_bench(steps=20000):
         ms = b.time(steps)
         hist = b.probes(i)
         rho, _, _ = b.moments(i)
+        mass, energy = b.totals(i)
         b.close()
         runs[name] = (hist, rho)
-        out[name] = {"s": round(ms / 1e3, 3), "MLUPS": round(8192 * 256 * steps / ms / 1e3, 1)}
+        out[name] = {"s": round(ms / 1e3, 3), "MLUPS": round(8192 * 256 * steps / ms / 1e3, 1),
+                     "mass": mass, "kinetic_energy": energy}
     cs2 = 1.0 / 3.0
     scale = 1e-3 * cs2                       # the pulse's initial p' amplitude
     h64, r64 = runs["fp64"]
@@ -478,6 +617,9 @@ def liner_bench(steps=20000):
         out[name]["wall_p_final_err"] = float(
             f"{np.abs(cs2 * (r[0, 3072:5121] - 1.0) - wall64).max() / scale:.3e}")
         out[name]["transmitted_err"] = float(f"{cs2 * np.abs(h[:, 3] - h64[:, 3]).max() / scale:.3e}")
+        for k in ("mass", "kinetic_energy"):
+            out[name][k + "_rel_diff"] = float(
+                f"{abs(out[name][k] - out['fp64'][k]) / abs(out['fp64'][k]):.3e}")
     out["transmitted_peak_over_incident"] = float(
         f"{cs2 * np.abs(h64[:, 3] - 1.0).max() / (scale / 2):.4f}")
     log(f"liner duct: {out}")
@@ -608,8 +750,10 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=5000)
     ap.add_argument("--batched-sample", type=int, default=48,
                     help="lattices timed one by one for the sequential FP64 baseline")
-    ap.add_argument("--serving", type=int, default=32, metavar="N",
+    ap.add_argument("--serving", type=int, default=128, metavar="N",
                     help="requests per arrival rate of the serving sweep (0: skip)")
+    ap.add_argument("--dropin", type=int, default=100, metavar="STEPS",
+                    help="main_lbm calls of the drop-in benchmark (0: skip)")
     ap.add_argument("--cylinder-steps", type=int, default=50000)
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",
                     help="skip the Taylor-Green roofline sweep 1024^2 .. 16384^2")
@@ -724,6 +868,8 @@ def main():
                                              args.batched_sample)
     if args.serving > 0:
         secondary["serving"] = serving_bench(n=args.serving)
+    if args.dropin > 0:
+        secondary["dropin"] = dropin_bench(args.dropin)
     if args.liner > 0:
         secondary["liner_duct"] = liner_bench(args.liner)
 
