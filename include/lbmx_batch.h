@@ -165,6 +165,18 @@ LBMX_API int lbmx_batch_get_tuning(lbmx_batch *b, lbmx_tuning *t);
 /* Steps per HIP-graph launch (default 32). */
 LBMX_API int lbmx_batch_set_chunk(lbmx_batch *b, int steps);
 
+/* Pair mode (on != 0): two time steps per pass over each lattice.  A lattice's
+ * core (nodes at least 3 from every edge) takes both steps in one launch, its
+ * intermediate step kept in LDS; the nodes nearer an edge, where every boundary
+ * rule lives, take one step per launch through a small frame buffer.  Results
+ * are bit-identical to one step per launch; HBM traffic per step about halves.
+ * The batch then advances in steps of two (lbmx_batch_run, lbmx_batch_time and
+ * the chunk must be even; a lattice with an odd `steps` takes a single step at
+ * the end), launch shapes are fixed (lbmx_batch_tune returns the current tuning)
+ * and the mode can change only before any lattice has started.  Off by default;
+ * a server with an even chunk uses it. */
+LBMX_API int lbmx_batch_set_pairs(lbmx_batch *b, int on);
+
 /* ---- server: continuous batching of solve requests ------------------------------
  * A worker thread owns a batch.  Submitted requests wait in arrival order and are
  * admitted whenever the memory budget can hold them; between graph chunks the
@@ -196,7 +208,8 @@ typedef struct {
 
 typedef struct lbmx_server lbmx_server;
 
-/* precision/budget as for a batch; chunk = steps per graph launch. */
+/* precision/budget as for a batch; chunk = steps per graph launch (an even chunk
+ * runs the batch in pair mode, lbmx_batch_set_pairs). */
 LBMX_API int lbmx_server_create(int precision, size_t budget, int chunk,
                                 const lbmx_tuning *tuning, lbmx_server **out);
 /* Queues one request (thread-safe, returns at once). */

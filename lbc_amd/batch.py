@@ -110,6 +110,7 @@ def _bind():
     L.lbmx_server_destroy.restype = None
     L.lbmx_batch_spectrum.argtypes = [vp, i, C.POINTER(TlSpec), C.POINTER(Result)]
     L.lbmx_batch_status.argtypes = [vp, i, C.POINTER(C.c_uint)]
+    L.lbmx_batch_set_pairs.argtypes = [vp, i]
     L.lbmx_batch_totals.argtypes = [vp, i, C.POINTER(d), C.POINTER(d)]
     _bound = True
     return L
@@ -441,6 +442,10 @@ class Batch:
 
     def set_chunk(self, steps):
         check(self._L.lbmx_batch_set_chunk(self.h, int(steps)))
+
+    def set_pairs(self, on=True):
+        """Two steps per pass (lbmx_batch_set_pairs); runs must then be even."""
+        check(self._L.lbmx_batch_set_pairs(self.h, int(bool(on))))
 
 
 class Server:

@@ -90,6 +90,10 @@ static BatchLayout layout_of(size_t esize, const lbmx_case *c)
         L.buf[i] = o;
         o += a256((size_t)kQ * L.plane * esize);
     }
+    // frame buffer of pair mode: the nodes within 3 of an edge
+    L.fplane = (unsigned)(((frame_nodes_of(c->Nx, c->Ny) + 63) / 64) * 64 + 64);
+    L.fbuf = o;
+    o += a256((size_t)kQ * L.fplane * esize);
     L.om = o;
     o += a256((size_t)c->Nx * 8);
     for (int i = 0; i < 3; i++) {
@@ -241,6 +245,8 @@ static void fill_desc(lbmx_batch *b, BatchSlot &S)
     d = BatchLat{};
     d.buf[0] = blk + L.buf[0];
     d.buf[1] = blk + L.buf[1];
+    d.fbuf = blk + L.fbuf;
+    d.fplane = L.fplane;
     d.om_col = (const double *)(blk + L.om);
     for (int i = 0; i < 3; i++) d.out[i] = c->keep_moments ? (double *)(blk + L.out[i]) : nullptr;
     d.saved[0] = (double *)(blk + L.saved[0]);
@@ -376,8 +382,32 @@ static void launch_step_t(lbmx_batch *b, int local)
                        b->tiles_dev, b->ctl_dev, local);
 }
 
+// One pair of steps (pair mode): band step 1, core steps 1-2, band step 2.
+template <typename T, bool W>
+static void launch_pair_t(lbmx_batch *b, int local)
+{
+    const dim3 gc(std::max(8, b->grid5)), gb(std::max(8, b->grid_band));
+    hipLaunchKernelGGL((k_pair_band<T, W, 0>), gb, dim3(kBandTile), 0, b->stream, b->lats_dev,
+                       b->tiles_dev, b->ctl_dev, local);
+    hipLaunchKernelGGL((k_pair_core<T, W>), gc, dim3(PairShape<T>::threads), 0, b->stream,
+                       b->lats_dev, b->tiles_dev, b->ctl_dev, local);
+    hipLaunchKernelGGL((k_pair_band<T, W, 1>), gb, dim3(kBandTile), 0, b->stream, b->lats_dev,
+                       b->tiles_dev, b->ctl_dev, local);
+}
+
 static int launch_step(lbmx_batch *b, int local)
 {
+    if (b->pairs) {
+        // local is the first step of a pair (even)
+        if (b->precision == LBMX_F64)
+            launch_pair_t<double, false>(b, local);
+        else if (b->precision == LBMX_F32_WIDE)
+            launch_pair_t<float, true>(b, local);
+        else
+            launch_pair_t<float, false>(b, local);
+        HIPCHK(hipGetLastError());
+        return LBMX_OK;
+    }
     if (b->precision == LBMX_F64)
         launch_step_t<double, false>(b, local);
     else if (b->precision == LBMX_F32_WIDE)
@@ -445,16 +475,30 @@ static int commit(lbmx_batch *b)
     std::vector<Tile> &T = b->tile_host;
     T.clear();
     const int tile = b->tun.tile;
-    for (int slot : b->order) {
-        const BatchSlot &S = b->slots[slot];
-        const int ni = interior_nodes(S.c.Nx, S.c.Ny);
-        for (int s0 = 0; s0 < ni; s0 += tile) T.push_back(Tile{slot, s0});
+    if (b->pairs) {
+        // core tiles (kPairTX x pair_ty nodes), then band tiles (kBandTile nodes)
+        for (int slot : b->order) {
+            const BatchSlot &S = b->slots[slot];
+            const int nt = core_tiles(S.c.Nx, S.c.Ny, pair_ty((int)b->esize));
+            for (int t = 0; t < nt; t++) T.push_back(Tile{slot, t});
+        }
+    } else {
+        for (int slot : b->order) {
+            const BatchSlot &S = b->slots[slot];
+            const int ni = interior_nodes(S.c.Nx, S.c.Ny);
+            for (int s0 = 0; s0 < ni; s0 += tile) T.push_back(Tile{slot, s0});
+        }
     }
     const int n_interior = (int)T.size();
     for (int slot : b->order) {
         const BatchSlot &S = b->slots[slot];
-        const int nf = frame_nodes(S.c.Nx, S.c.Ny);
-        for (int f0 = 0; f0 < nf; f0 += kFrameTile) T.push_back(Tile{slot, f0});
+        if (b->pairs) {
+            const int nb = band_nodes(S.c.Nx, S.c.Ny);
+            for (int b0 = 0; b0 < nb; b0 += kBandTile) T.push_back(Tile{slot, b0});
+        } else {
+            const int nf = frame_nodes(S.c.Nx, S.c.Ny);
+            for (int f0 = 0; f0 < nf; f0 += kFrameTile) T.push_back(Tile{slot, f0});
+        }
     }
     if ((long long)T.size() > b->tiles_cap)
         return lbmx_fail(LBMX_ERR_NOMEM, "tile table overflow (%zu > %lld)", T.size(), b->tiles_cap);
@@ -462,9 +506,12 @@ static int commit(lbmx_batch *b)
     BatchCtl ctl{b->epoch, n_interior, (int)T.size() - n_interior};
     if (int rc = upload(b, b->ctl_dev, &ctl, sizeof ctl)) return rc;
     const int g5 = grid5_of(n_interior);
-    if (b->tun.order == 5 && g5 != b->grid5) drop_graphs(b);
+    const int gb = grid5_of((int)T.size() - n_interior);
+    if ((b->tun.order == 5 || b->pairs) && g5 != b->grid5) drop_graphs(b);
+    if (b->pairs && gb != b->grid_band) drop_graphs(b);
     b->n_interior = n_interior;
     b->grid5 = g5;
+    b->grid_band = gb;
     b->dirty = false;
     return LBMX_OK;
 }
@@ -479,7 +526,8 @@ static int graph_for(lbmx_batch *b, int K, hipGraphExec_t *out)
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int rc = LBMX_OK;
-    for (int l = 0; l < K && rc == LBMX_OK; l++) rc = launch_step(b, l);
+    const int per = b->pairs ? 2 : 1;
+    for (int l = 0; l < K && rc == LBMX_OK; l += per) rc = launch_step(b, l);
     if (rc == LBMX_OK) rc = launch_advance(b, K);
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc) {
@@ -498,6 +546,8 @@ static int graph_for(lbmx_batch *b, int K, hipGraphExec_t *out)
 
 int lbmx_batch_enqueue(lbmx_batch *b, int nsteps)
 {
+    if (b->pairs && (nsteps & 1))
+        return lbmx_fail(LBMX_ERR_ARG, "pair mode advances in steps of two (asked for %d)", nsteps);
     if (int rc = commit(b)) return rc;
     const int K = b->chunk;
     const int full = (b->use_graphs && K > 1) ? nsteps / K : 0;
@@ -507,7 +557,7 @@ int lbmx_batch_enqueue(lbmx_batch *b, int nsteps)
         for (int i = 0; i < full; i++) HIPCHK(hipGraphLaunch(ex, b->stream));
     }
     const int rem = nsteps - full * K;
-    for (int l = 0; l < rem; l++)
+    for (int l = 0; l < rem; l += b->pairs ? 2 : 1)
         if (int rc = launch_step(b, l)) return rc;
     if (rem > 0)
         if (int rc = launch_advance(b, rem)) return rc;
@@ -890,9 +940,26 @@ LBMX_API int lbmx_batch_get_tuning(lbmx_batch *b, lbmx_tuning *t)
     return LBMX_OK;
 }
 
+LBMX_API int lbmx_batch_set_pairs(lbmx_batch *b, int on)
+{
+    if (!b) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    const bool want = on != 0;
+    if (want == b->pairs) return LBMX_OK;
+    for (int slot : b->order)
+        if (!b->slots[slot].pending)
+            return lbmx_fail(LBMX_ERR_STATE, "the step mode can change only before lattices start");
+    if (want && ((b->epoch & 1) || (b->use_graphs && (b->chunk & 1))))
+        return lbmx_fail(LBMX_ERR_STATE, "pair mode needs an even epoch and an even chunk");
+    b->pairs = want;
+    drop_graphs(b);
+    b->dirty = true;
+    return LBMX_OK;
+}
+
 LBMX_API int lbmx_batch_set_chunk(lbmx_batch *b, int steps)
 {
     if (!b || steps < 1 || steps > 4096) return lbmx_fail(LBMX_ERR_ARG, "chunk must be in [1, 4096]");
+    if (b->pairs && (steps & 1)) return lbmx_fail(LBMX_ERR_ARG, "pair mode needs an even chunk");
     if (steps != b->chunk) drop_graphs(b);
     b->chunk = steps;
     b->use_graphs = steps > 1;
@@ -902,6 +969,10 @@ LBMX_API int lbmx_batch_set_chunk(lbmx_batch *b, int steps)
 LBMX_API int lbmx_batch_tune(lbmx_batch *b, int trial_steps, lbmx_tuning *chosen)
 {
     if (!b || trial_steps < 1) return lbmx_fail(LBMX_ERR_ARG, "invalid argument");
+    if (b->pairs) {   // the pair kernels have fixed shapes; the tuning is not used
+        if (chosen) *chosen = b->tun;
+        return LBMX_OK;
+    }
     for (int slot : b->order)
         if (!b->slots[slot].pending)
             return lbmx_fail(LBMX_ERR_STATE, "tuning needs lattices that have not started");

@@ -25,8 +25,8 @@ struct BatchArena {
 
 // Byte offsets of one lattice's arrays inside its arena block.
 struct BatchLayout {
-    size_t buf[2], om, out[3], saved[2], state, probe, init_a, init_b, total;
-    unsigned plane;
+    size_t buf[2], fbuf, om, out[3], saved[2], state, probe, init_a, init_b, total;
+    unsigned plane, fplane;
     int gap;
 };
 
@@ -68,6 +68,10 @@ struct lbmx_batch {
     lbmx_tuning tun{256, 512, 16, 5};
     int chunk = 32;
     bool use_graphs = true;
+    // pair mode: two steps per pass (k_pair_band / k_pair_core); the epoch then
+    // advances in steps of two
+    bool pairs = false;
+    int grid_band = 0;   // band-tile grid of pair mode (rounded like grid5)
     std::map<int, hipGraphExec_t> graphs;
     // graphs replaced while launches of them may still be queued on the stream:
     // destroyed once the stream is known to be idle

@@ -52,6 +52,8 @@ struct BatchLat {
     double *state;           // liner: phi[2] psi[2] chi[2] rho_lag (8 doubles) per opening node
     double *probe_hist;      // (steps + 1) x nprobe
     unsigned *status;        // divergence guard bits (kGuard*), cleared by k_batch_init
+    void *fbuf;              // frame buffer (T[9][fplane]): nodes within 3 of an edge (pairs)
+    unsigned fplane;
     long long start;         // batch epoch at which this lattice takes its step 0
     unsigned plane;
     int Nx, Ny;
@@ -163,19 +165,92 @@ __device__ __forceinline__ T b_enc(double v, int k, double rho0)
         return (float)(v - D2Q9::w[k] * rho0);
 }
 
+__device__ __forceinline__ void b_store(gptr<double> p, double v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+__device__ __forceinline__ void b_store(gptr<float> p, float v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
+// ---- where a step reads and writes populations -------------------------------------
+// A view addresses population k of node (x, y) in some buffer of one lattice.
+// FullV: a dense population buffer (nine planes of Nx*Ny nodes).  FrameV: the
+// lattice's frame buffer, which holds only the nodes within 3 of an edge (rows
+// 0-3 and Ny-4..Ny-1 whole, columns 0-3 and Nx-4..Nx-1 of the rows between; the
+// whole lattice when Nx or Ny is below 9).  One step reads its sources through
+// one view, the stale entries of a node's own slots through another and stores
+// through a third (the single-step kernels: f*_s, and f*_{s+1}'s buffer for both
+// the stale reads and the stores).
+template <typename T>
+struct FullV {
+    gptr<T> p;
+    unsigned P;
+    int Nx;
+    __device__ __forceinline__ T at(int k, int x, int y) const
+    {
+        return p[k * P + (unsigned)(y * Nx + x)];
+    }
+    __device__ __forceinline__ void put(int k, int x, int y, T v) const
+    {
+        b_store(&p[k * P + (unsigned)(y * Nx + x)], v);
+    }
+};
+
+__host__ __device__ __forceinline__ bool frame_is_full(int Nx, int Ny)
+{
+    return Nx < 9 || Ny < 9;
+}
+__host__ __device__ __forceinline__ unsigned frame_nodes_of(int Nx, int Ny)
+{
+    return frame_is_full(Nx, Ny) ? (unsigned)(Nx * Ny) : (unsigned)(8 * Nx + 8 * (Ny - 8));
+}
+// Index of node (x, y) in the frame buffer (the node must lie within 3 of an edge).
+__device__ __forceinline__ unsigned frame_index(int Nx, int Ny, int x, int y)
+{
+    if (frame_is_full(Nx, Ny) || y < 4) return (unsigned)(y * Nx + x);
+    if (y >= Ny - 4) return (unsigned)((y - Ny + 8) * Nx + x);
+    return (unsigned)(8 * Nx + (y - 4) * 8 + (x < 4 ? x : x - Nx + 8));
+}
+
+template <typename T>
+struct FrameV {
+    gptr<T> p;
+    unsigned P;
+    int Nx, Ny;
+    __device__ __forceinline__ T at(int k, int x, int y) const
+    {
+        return p[k * P + frame_index(Nx, Ny, x, y)];
+    }
+    __device__ __forceinline__ void put(int k, int x, int y, T v) const
+    {
+        p[k * P + frame_index(Nx, Ny, x, y)] = v;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ FullV<T> full_view(const BatchLat &L, int b)
+{
+    return FullV<T>{gbl((T *)L.buf[b]), L.plane, L.Nx};
+}
+template <typename T>
+__device__ __forceinline__ FrameV<T> frame_view(const BatchLat &L)
+{
+    return FrameV<T>{gbl((T *)L.fbuf), L.fplane, L.Nx, L.Ny};
+}
+
 // Pre-collision populations of node (x, y): pull from x - c_k with the wrap,
 // bounce-back and stale rules above.  Stale entries are read from the node's own
-// slots of D, which only this node's thread writes (callers gathering for another
-// node never meet a stale entry; the host validates that).
-template <typename T>
-__device__ __forceinline__ void b_gather(const BatchLat &L, gptr<const T> __restrict__ S,
-                                         gptr<const T> D, int x, int y, T f[Q])
+// slots through O, which only this node's thread writes (callers gathering for
+// another node never meet a stale entry; the host validates that).
+template <typename T, class SV, class OV>
+__device__ __forceinline__ void b_gather(const BatchLat &L, const SV &S, const OV &O, int x, int y,
+                                         T f[Q])
 {
     const int Nx = L.Nx, Ny = L.Ny;
-    const unsigned P = L.plane;
-    const unsigned nc = (unsigned)(y * Nx + x);
     const bool px = L.flags & BF_PER_X, py = L.flags & BF_PER_Y;
-    f[0] = S[nc];
+    f[0] = S.at(0, x, y);
 #pragma unroll
     for (int k = 1; k < Q; k++) {
         const int cx = D2Q9::cx[k], cy = D2Q9::cy[k];
@@ -200,31 +275,22 @@ __device__ __forceinline__ void b_gather(const BatchLat &L, gptr<const T> __rest
             }
         }
         const bool bb = !oob && b_solid(L, xs, ys);
-        const unsigned idx = oob ? k * P + nc
-                                 : (bb ? D2Q9::opp[k] * P + nc : k * P + (unsigned)(ys * Nx + xs));
-        f[k] = oob ? D[idx] : S[idx];
+        f[k] = oob ? O.at(k, x, y) : (bb ? S.at(D2Q9::opp[k], x, y) : S.at(k, xs, ys));
     }
 }
 
-__device__ __forceinline__ void b_store(gptr<double> p, double v)
-{
-    __builtin_nontemporal_store(v, p);
-}
-__device__ __forceinline__ void b_store(gptr<float> p, float v)
-{
-    __builtin_nontemporal_store(v, p);
-}
-
-// Moments, collision, outputs and store of node (x, y) whose pre-collision
-// populations (after its boundary closures) are f.
+// Moments, collision and the node's outputs for node (x, y) whose pre-collision
+// populations (after its boundary closures) are f, which then holds f*_{s+1}.
+// own = false (a neighbour recomputed by another tile): the collision only.
 template <typename T, bool W>
-__device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, int y, int n,
-                                             T f[Q], gptr<T> __restrict__ D, bool last)
+__device__ __forceinline__ void batch_update(const BatchLat &L, int s, int x, int y, int n, T f[Q],
+                                             bool last, bool own = true)
 {
     const unsigned F = L.flags;
     const double rho0 = L.rho0;
     const bool force = F & BF_FORCE;
     const double om = gbl(L.om_col)[x];
+    const bool moments = own && last && (F & BF_MOMENTS);
     double rl, ul, vl;   // next step's lagged moments as the reference's arrays hold them
     if constexpr (sizeof(T) == 8) {
         double *fd = reinterpret_cast<double *>(f);
@@ -232,7 +298,7 @@ __device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, in
         rl = m.rho;
         ul = m.ux;
         vl = m.uy;
-        if (last && (F & BF_MOMENTS)) {
+        if (moments) {
             gbl(L.out[0])[n] = m.rho;
             gbl(L.out[1])[n] = m.ux;
             gbl(L.out[2])[n] = m.uy;
@@ -244,7 +310,7 @@ __device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, in
         rl = rho0 + m.dr;
         ul = m.ux;
         vl = m.uy;
-        if (last && (F & BF_MOMENTS)) {
+        if (moments) {
             double fd[Q];
 #pragma unroll
             for (int k = 0; k < Q; k++) fd[k] = b_dec(q[k], k, rho0);
@@ -261,7 +327,7 @@ __device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, in
         rl = (double)m.dr + rho0;
         ul = (double)m.ux;
         vl = (double)m.uy;
-        if (last && (F & BF_MOMENTS)) {
+        if (moments) {
             double fd[Q];
 #pragma unroll
             for (int k = 0; k < Q; k++) fd[k] = b_dec(q[k], k, rho0);
@@ -272,6 +338,7 @@ __device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, in
         }
         collide_f32(q, m, r0, (float)om, force, (float)L.Fx, (float)L.Fy);
     }
+    if (!own) return;
 
     guard_check(L, rl, ul, vl);
 
@@ -286,10 +353,6 @@ __device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, in
     }
     for (int i = 0; i < L.nprobe; i++)
         if (n == L.probe_n[i]) gbl(L.probe_hist)[(long long)(s + 1) * L.nprobe + i] = rl;
-
-    const unsigned P = L.plane;
-#pragma unroll
-    for (int k = 0; k < Q; k++) b_store(&D[k * P + (unsigned)n], f[k]);
 }
 
 // Node (x, y) of rows 1 .. Ny-2.  Away from the edge columns (2 <= x <= Nx-3,
@@ -299,20 +362,17 @@ __device__ __forceinline__ void batch_finish(const BatchLat &L, int s, int x, in
 // takes its inner neighbour's populations; no other closure touches these rows.
 // Every lane stores all nine slots (a solid node stores zeros: its slots are
 // never read), so a wavefront writes whole cache lines.
-template <typename T, bool W>
-__device__ __forceinline__ void batch_row_node(const BatchLat &L, gptr<const T> __restrict__ S,
-                                               gptr<T> __restrict__ D, int s, int x, int y,
-                                               bool last)
+template <typename T, bool W, class SV, class OV, class DV>
+__device__ __forceinline__ void batch_row_node(const BatchLat &L, const SV &S, const OV &O,
+                                               const DV &D, int s, int x, int y, bool last)
 {
     const int Nx = L.Nx, Ny = L.Ny;
     const int n = y * Nx + x;
-    const unsigned P = L.plane;
     const unsigned F = L.flags;
     T f[Q];
     if ((unsigned)(x - 2) < (unsigned)(Nx - 4) && (unsigned)(y - 2) < (unsigned)(Ny - 4)) {
 #pragma unroll
-        for (int k = 0; k < Q; k++)
-            f[k] = S[k * P + (unsigned)(n - D2Q9::cy[k] * Nx - D2Q9::cx[k])];
+        for (int k = 0; k < Q; k++) f[k] = S.at(k, x - D2Q9::cx[k], y - D2Q9::cy[k]);
     } else {
         if (b_solid(L, x, y)) {
             if (last && (F & BF_MOMENTS)) {
@@ -321,20 +381,23 @@ __device__ __forceinline__ void batch_row_node(const BatchLat &L, gptr<const T> 
                 gbl(L.out[2])[n] = NAN;
             }
 #pragma unroll
-            for (int k = 0; k < Q; k++) b_store(&D[k * P + (unsigned)n], (T)0);
+            for (int k = 0; k < Q; k++) D.put(k, x, y, (T)0);
             return;
         }
         int gx = x;
         if ((F & BF_CONVECTIVE) && (x == 0 || x == Nx - 1)) gx = (x == 0) ? 1 : Nx - 2;
-        b_gather<T>(L, S, D, gx, y, f);
+        b_gather<T>(L, S, O, gx, y, f);
     }
-    batch_finish<T, W>(L, s, x, y, n, f, D, last);
+    batch_update<T, W>(L, s, x, y, n, f, last);
+#pragma unroll
+    for (int k = 0; k < Q; k++) D.put(k, x, y, f[k]);
 }
 
 // Node of row 0 or Ny-1: wrap, walls, stale entries and the case's boundary
 // closures (moving lid, impedance wall, convective corners).
-template <typename T, bool W>
-__device__ __forceinline__ void batch_frame(const BatchLat &L, int s, int x, int y)
+template <typename T, bool W, class SV, class OV, class DV>
+__device__ __forceinline__ void batch_frame(const BatchLat &L, const SV &S, const OV &O,
+                                            const DV &D, int s, int x, int y)
 {
     const int Nx = L.Nx, Ny = L.Ny;
     const int n = y * Nx + x;
@@ -348,8 +411,6 @@ __device__ __forceinline__ void batch_frame(const BatchLat &L, int s, int x, int
         }
         return;
     }
-    const gptr<const T> S = gbl((const T *)L.buf[s & 1]);
-    const gptr<T> D = gbl((T *)L.buf[(s + 1) & 1]);
     const double rho0 = L.rho0;
 
     // convective_{inlet,outlet}: the node takes the post-streaming populations of
@@ -361,7 +422,7 @@ __device__ __forceinline__ void batch_frame(const BatchLat &L, int s, int x, int
     const bool liner = (F & BF_LINER) && y == 0 && x >= L.gap_x0 && x <= L.gap_x1;
     if (liner) gy = 1;
     T f[Q];
-    b_gather<T>(L, S, D, gx, gy, f);
+    b_gather<T>(L, S, O, gx, gy, f);
 
     if (liner) {
         // cases/nit/nit.c:13-47 per node, then velocity_bottom_regularized.c:22-70
@@ -401,7 +462,9 @@ __device__ __forceinline__ void batch_frame(const BatchLat &L, int s, int x, int
         f[7] = b_enc<T>(fd[7], 7, rho0);
         f[8] = b_enc<T>(fd[8], 8, rho0);
     }
-    batch_finish<T, W>(L, s, x, y, n, f, D, last);
+    batch_update<T, W>(L, s, x, y, n, f, last);
+#pragma unroll
+    for (int k = 0; k < Q; k++) D.put(k, x, y, f[k]);
 }
 
 // Node counts of the two tile kinds of a lattice (Nx, Ny >= 4): row tiles run over
@@ -476,13 +539,12 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_interior(
         if (s < 0 || s >= L.steps) continue;
         const int Nx = L.Nx;
         const int end = min(tl.start + tile, interior_nodes(Nx, L.Ny));
-        const gptr<const T> S = gbl((const T *)L.buf[s & 1]);
-        const gptr<T> D = gbl((T *)L.buf[(s + 1) & 1]);
+        const FullV<T> S = full_view<T>(L, (int)(s & 1)), D = full_view<T>(L, (int)((s + 1) & 1));
         const bool last = (s + 1 == L.steps);
         for (int m = tl.start + (int)threadIdx.x; m < end; m += (int)blockDim.x) {
             // exact for m < 2^26: (m + 1/2)/Nx is >= 1/(2 Nx) away from an integer
             const int r = (int)(((double)m + 0.5) * L.inv_nx);
-            batch_row_node<T, W>(L, S, D, (int)s, m - r * Nx, 1 + r, last);
+            batch_row_node<T, W>(L, S, D, D, (int)s, m - r * Nx, 1 + r, last);
         }
     }
 }
@@ -510,8 +572,199 @@ __global__ void __launch_bounds__(kFrameThreads) k_batch_frame(
         const int end = min(tl.start + kFrameTile, frame_nodes(Nx, L.Ny));
         for (int f = tl.start + (int)threadIdx.x; f < end; f += (int)blockDim.x) {
             const int top = f >= Nx;
-            batch_frame<T, W>(L, (int)s, f - top * Nx, top ? L.Ny - 1 : 0);
+            const FullV<T> S = full_view<T>(L, (int)(s & 1)), D = full_view<T>(L, (int)((s + 1) & 1));
+            batch_frame<T, W>(L, S, D, D, (int)s, f - top * Nx, top ? L.Ny - 1 : 0);
         }
+    }
+}
+
+// ---- two steps per pass (pair mode) -------------------------------------------------
+// A lattice's nodes split into the core -- 3 <= x <= Nx-4 and 3 <= y <= Ny-4, where
+// a node and every source of its gather obey the plain rule -- and the band, the
+// nodes within 2 of an edge, where every boundary rule and closure lives.  With
+// the state f*_s in buf[X], X = (s / 2) & 1 (pairs start at even s), and Y = 1 - X:
+//   k_pair_band<0>  step s+1 of the band nodes: sources buf[X], stale entries
+//                   (f*_{s-1}) from the frame buffer's own slots, stores to the
+//                   frame buffer;
+//   k_pair_core     step s+1 of a core tile and a one-node ring in LDS (the ring
+//                   nodes are plain), the tile's nodes within 3 of an edge also to
+//                   the frame buffer, then step s+2 of the tile, stored to buf[Y];
+//   k_pair_band<1>  step s+2 of the band nodes: sources from the frame buffer
+//                   (band nodes and the core's outer ring), stale entries (f*_s)
+//                   from buf[X]'s own slots, stores to buf[Y].
+// The frame buffer then again holds f*_{s+1} at the band nodes, the stale values
+// of the next pair.  Each node of each step is computed once by its owner with the
+// arithmetic of the single-step kernels, so results are bit-identical to them.
+// A lattice whose last step is s+1 takes one step: the first band phase and the
+// core kernel store f*_{s+1} to buf[Y] and the second band phase skips it.
+constexpr int kPairTX = 64;
+// core tile rows and workgroup size by stored type: FP32 storage 64 x 16 tiles on
+// 512 lanes (42.8 KB of LDS, six waves per SIMD; 64 x 8 on 256 lanes recomputes
+// 29 % of the nodes instead of 16 %), FP64 64 x 8 on 256 lanes (47.5 KB)
+template <typename T> struct PairShape {
+    static constexpr int TY = sizeof(T) == 8 ? 8 : 16;
+    static constexpr int threads = sizeof(T) == 8 ? 256 : 512;
+};
+__host__ __device__ __forceinline__ int pair_ty(int esize) { return esize == 8 ? 8 : 16; }
+constexpr int kBandTile = 256;
+
+__host__ __device__ __forceinline__ int core_w(int Nx) { return Nx > 6 ? Nx - 6 : 0; }
+__host__ __device__ __forceinline__ int core_h(int Ny) { return Ny > 6 ? Ny - 6 : 0; }
+__host__ __device__ __forceinline__ int band_nodes(int Nx, int Ny)
+{
+    return Nx * Ny - core_w(Nx) * core_h(Ny);
+}
+// Core tiles cover columns [64 t, 64 t + 64) (cut to the core), so that a
+// tile's rows start on a 128-byte line when Nx is a multiple of 32, and rows
+// [3 + ty j, 3 + ty (j + 1)).
+__host__ __device__ __forceinline__ int core_tiles_x(int Nx)
+{
+    return core_w(Nx) ? (Nx - 4 + kPairTX - 1) / kPairTX : 0;
+}
+__host__ __device__ __forceinline__ int core_tiles(int Nx, int Ny, int ty)
+{
+    return core_tiles_x(Nx) * ((core_h(Ny) + ty - 1) / ty);
+}
+
+// Workgroup -> item of a table of n items: XCD i (workgroups with blockIdx % 8 ==
+// i) takes the i-th eighth of the table in order, so neighbouring tiles -- whose
+// halo rows overlap -- are read through the same L2.  The grid is a multiple of 8.
+__device__ __forceinline__ int xcd_item()
+{
+    const int per = (int)gridDim.x / 8;
+    return ((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8;
+}
+
+// Band node m (row-major over the band) -> (x, y).
+__device__ __forceinline__ void band_node(int Nx, int Ny, int m, int &x, int &y)
+{
+    const int ch = core_h(Ny);
+    if (core_w(Nx) == 0 || ch == 0) {
+        y = m / Nx;
+        x = m - y * Nx;
+        return;
+    }
+    if (m < 3 * Nx) {
+        y = m / Nx;
+        x = m - y * Nx;
+    } else if (m < 3 * Nx + 6 * ch) {
+        const int q = m - 3 * Nx, r = q / 6, c = q - 6 * r;
+        y = 3 + r;
+        x = c < 3 ? c : Nx - 6 + c;
+    } else {
+        const int q = m - 3 * Nx - 6 * ch, r = q / Nx;
+        y = Ny - 3 + r;
+        x = q - r * Nx;
+    }
+}
+
+template <typename T, bool W, int TY = PairShape<T>::TY, int NT = PairShape<T>::threads>
+__global__ void __launch_bounds__(NT) k_pair_core(const BatchLat *__restrict__ lats,
+                                                           const Tile *__restrict__ tiles,
+                                                           const BatchCtl *__restrict__ ctl,
+                                                           int local)
+{
+    constexpr int RX = kPairTX + 2, RN = RX * (TY + 2);
+    __shared__ T buf[Q * RN];
+    const long long epoch = ctl->epoch + local;
+    const int item = xcd_item();
+    if (item >= ctl->n_interior) return;
+    const Tile tl = tiles[item];
+    const BatchLat &L = lats[tl.lat];
+    const long long s = epoch - L.start;
+    if (s < 0 || s >= L.steps) return;
+    const bool single = (s + 1 == L.steps);
+    const int Nx = L.Nx, Ny = L.Ny;
+    const int ntx = core_tiles_x(Nx);
+    const int x0 = (tl.start % ntx) * kPairTX, y0 = 3 + (tl.start / ntx) * TY;
+    const int X = (int)((s >> 1) & 1);
+    const FullV<T> S = full_view<T>(L, X), D = full_view<T>(L, X ^ 1);
+    const FrameV<T> R = frame_view<T>(L);
+
+    // step s+1: the tile and its one-node ring (plain nodes: 2 <= x <= Nx-3)
+    for (int r = threadIdx.x; r < RN; r += NT) {
+        const int lx = r % RX, ly = r / RX;
+        const int x = x0 - 1 + lx, y = y0 - 1 + ly;
+        // only plain nodes (2 <= x <= Nx-3) are read by the tile's core nodes
+        if (x < 2 || x > Nx - 3 || y > Ny - 3) continue;
+        const bool own = lx >= 1 && lx <= kPairTX && ly >= 1 && ly <= TY && x >= 3 &&
+                         x <= Nx - 4 && y <= Ny - 4;
+        if (single && !own) continue;
+        T p[Q];
+#pragma unroll
+        for (int k = 0; k < Q; k++) p[k] = S.at(k, x - D2Q9::cx[k], y - D2Q9::cy[k]);
+        batch_update<T, W>(L, (int)s, x, y, y * Nx + x, p, single, own);
+        if (single) {
+#pragma unroll
+            for (int k = 0; k < Q; k++) D.put(k, x, y, p[k]);
+            continue;
+        }
+        if (own && (x == 3 || y == 3 || x == Nx - 4 || y == Ny - 4)) {
+#pragma unroll
+            for (int k = 0; k < Q; k++) R.put(k, x, y, p[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < Q; k++) buf[k * RN + r] = p[k];
+    }
+    if (single) return;   // the whole workgroup: the lattice's last step was this one
+    __syncthreads();
+
+    // step s+2: the tile, from LDS
+    const bool last = (s + 2 == L.steps);
+    for (int q = threadIdx.x; q < kPairTX * TY; q += NT) {
+        const int tx = q % kPairTX, ty = q / kPairTX;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x < 3 || x > Nx - 4 || y > Ny - 4) continue;
+        T p[Q];
+#pragma unroll
+        for (int k = 0; k < Q; k++)
+            p[k] = buf[k * RN + (ty + 1 - D2Q9::cy[k]) * RX + (tx + 1 - D2Q9::cx[k])];
+        batch_update<T, W>(L, (int)s + 1, x, y, y * Nx + x, p, last);
+#pragma unroll
+        for (int k = 0; k < Q; k++) D.put(k, x, y, p[k]);
+    }
+}
+
+// Band node (x, y) with the views of one phase (frame rows, or rows 1 .. Ny-2).
+template <typename T, bool W, class SV, class OV, class DV>
+__device__ __forceinline__ void band_step(const BatchLat &L, const SV &S, const OV &O, const DV &D,
+                                          int s, int x, int y)
+{
+    if (y == 0 || y == L.Ny - 1)
+        batch_frame<T, W>(L, S, O, D, s, x, y);
+    else
+        batch_row_node<T, W>(L, S, O, D, s, x, y, s + 1 == L.steps);
+}
+
+template <typename T, bool W, int PHASE>
+__global__ void __launch_bounds__(kBandTile) k_pair_band(const BatchLat *__restrict__ lats,
+                                                        const Tile *__restrict__ tiles,
+                                                        const BatchCtl *__restrict__ ctl,
+                                                        int local)
+{
+    const long long epoch = ctl->epoch + local;
+    const int item = xcd_item();
+    if (item >= ctl->n_frame) return;
+    const Tile tl = tiles[ctl->n_interior + item];
+    const BatchLat &L = lats[tl.lat];
+    const long long s = epoch - L.start;
+    if (s < 0 || s >= L.steps) return;
+    const bool single = (s + 1 == L.steps);
+    if (PHASE == 1 && single) return;
+    const int m = tl.start + (int)threadIdx.x;
+    if (m >= band_nodes(L.Nx, L.Ny)) return;
+    int x, y;
+    band_node(L.Nx, L.Ny, m, x, y);
+    const int X = (int)((s >> 1) & 1);
+    const FullV<T> A = full_view<T>(L, X), B = full_view<T>(L, X ^ 1);
+    const FrameV<T> R = frame_view<T>(L);
+    if (PHASE == 0) {
+        if (single)
+            band_step<T, W>(L, A, R, B, (int)s, x, y);
+        else
+            band_step<T, W>(L, A, R, R, (int)s, x, y);
+    } else {
+        band_step<T, W>(L, R, A, B, (int)s + 1, x, y);
     }
 }
 
@@ -561,6 +814,14 @@ __global__ void __launch_bounds__(256) k_batch_init(const BatchLat *__restrict__
                                   uy, usq);
         q[k] = b_enc<T>(fe, k, rho0);
         B[k * P + n] = q[k];
+    }
+    // the frame buffer's nodes: the same stale values for pair mode's first step
+    if (L.fbuf && (x < 4 || y < 4 || x >= L.Nx - 4 || y >= L.Ny - 4 ||
+                   frame_is_full(L.Nx, L.Ny))) {
+        T *Rb = (T *)L.fbuf;
+        const unsigned ri = frame_index(L.Nx, L.Ny, x, y);
+#pragma unroll
+        for (int k = 0; k < Q; k++) Rb[k * L.fplane + ri] = q[k];
     }
     const bool solid = b_solid(L, x, y);
     if (!solid) {

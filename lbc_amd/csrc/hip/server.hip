@@ -237,6 +237,7 @@ LBMX_API int lbmx_server_create(int precision, size_t budget, int chunk,
     s->t0 = std::chrono::steady_clock::now();
     int rc = lbmx_batch_create(precision, budget, &s->b);
     if (!rc) rc = lbmx_batch_set_chunk(s->b, chunk);
+    if (!rc && !(chunk & 1)) rc = lbmx_batch_set_pairs(s->b, 1);
     if (!rc && tuning) rc = lbmx_batch_set_tuning(s->b, tuning);
     if (!rc && hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess)
         rc = lbmx_fail(LBMX_ERR_HIP, "aux stream");
