@@ -317,6 +317,24 @@ def liner_duct(Nx, Ny, steps, impedance, tau=0.51, amp=1e-3, x0=None, halfwidth=
                 kind="liner")
 
 
+def impedance_tube(Nx, Ny, steps, impedance, y0, halfwidth, probe_y, amp=1e-4, tau=0.51,
+                   gain=-1.0):
+    """Normal incidence on an impedance wall: periodic in x, the whole bottom row is
+    the impedance wall (impedance None: a solid row, i.e. a rigid wall), rigid top
+    row, and a Gaussian plane pulse centred at y0 travelling toward the bottom
+    (rho' = amp g(y), v = -cs rho' / rho0).  Returns (case, fields) for
+    Batch.add(case, fields)."""
+    liner = impedance is not None
+    c = Case(Nx, Ny, steps, tau=tau, per_x=1, walls=WALL_BOTTOM | WALL_TOP,
+             gap=(0, Nx - 1) if liner else (0, -1), liner=impedance, gain=gain,
+             init=INIT_FIELDS, probes=[(Nx // 2, probe_y)], kind="tube")
+    y = np.arange(Ny, dtype=np.float64)
+    g = amp * np.exp(-math.log(2.0) * ((y - y0) / halfwidth) ** 2)
+    rho = np.ascontiguousarray(np.broadcast_to((1.0 + g)[:, None], (Ny, Nx)))
+    v = np.ascontiguousarray(np.broadcast_to((-CS * g)[:, None], (Ny, Nx)))
+    return c, (rho, np.zeros((Ny, Nx)), v)
+
+
 def batched_workload(n=1024, steps=5000, seed=2024):
     """The batched benchmark set: sizes uniform in {64..1024} x {64..512} (multiples
     of 32), family uniform over Poiseuille (tau in [0.6, 1.2]), Couette, lid-driven

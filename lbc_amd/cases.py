@@ -208,6 +208,33 @@ def ade_stable(imp, margin=1e-9):
     return True
 
 
+def ade_admittance(imp, w):
+    """Y(i w) of the impedance ODE in continuous time (nit.c:19-45 with dt -> 0):
+    v = Zinf p - sum A phi - 2 sum (B psi + C chi), phi' = p + lambda phi,
+    psi' = p + alpha psi - beta chi, chi' = beta psi + alpha chi.  For p = e^{st}:
+    Phi = 1/(s - lambda), Psi = (s - alpha)/D, Chi = beta/D, D = (s - alpha)^2 + beta^2."""
+    s = 1j * np.asarray(w, dtype=np.float64)
+    Y = np.full(s.shape, complex(imp["Zinf"]))
+    for i in range(2):
+        if imp["A"][i] != 0.0:
+            Y = Y - imp["A"][i] / (s - imp["lambdas"][i])
+        if imp["B"][i] != 0.0 or imp["C"][i] != 0.0:
+            a, b = imp["alpha"][i], imp["beta"][i]
+            D = (s - a) ** 2 + b * b
+            Y = Y - 2.0 * (imp["B"][i] * (s - a) + imp["C"][i] * b) / D
+    return Y
+
+
+def normal_reflection(imp, w, rho0=1.0, gain=-1.0):
+    """Plane-wave reflection coefficient of an impedance wall at normal incidence,
+    R = (Z - rho0 c0) / (Z + rho0 c0) = (1 - rho0 c0 Y) / (1 + rho0 c0 Y), where Y
+    relates the velocity INTO the wall to p'.  The nit tube's outlet takes the ODE
+    velocity as u, toward its wall (nit.c:302-303); the liner's bottom wall imposes
+    v_y = gain * v_ODE, so the velocity into it is -gain * v_ODE."""
+    Y = -gain * ade_admittance(imp, w)
+    return (1.0 - rho0 * CS * Y) / (1.0 + rho0 * CS * Y)
+
+
 @dataclass
 class LinerSpec:
     Nx: int = 8192

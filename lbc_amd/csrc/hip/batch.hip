@@ -196,8 +196,11 @@ static int validate(const lbmx_case *c)
     if (c->liner) {
         if (!(c->walls & LBMX_WALL_BOTTOM) || c->per_y)
             return lbmx_fail(LBMX_ERR_ARG, "the impedance wall is an opening of a bottom wall");
-        if (c->gap_x1 < c->gap_x0 || c->gap_x0 < 1 || c->gap_x1 > c->Nx - 2)
-            return lbmx_fail(LBMX_ERR_ARG, "impedance opening must lie inside [1, Nx-2]");
+        // a non-periodic row keeps its end nodes solid (corners of the side walls or
+        // of convective ends); a periodic row may be an impedance wall end to end
+        const int lo = c->per_x ? 0 : 1, hi = c->per_x ? c->Nx - 1 : c->Nx - 2;
+        if (c->gap_x1 < c->gap_x0 || c->gap_x0 < lo || c->gap_x1 > hi)
+            return lbmx_fail(LBMX_ERR_ARG, "impedance opening must lie inside [%d, %d]", lo, hi);
         for (int x = c->gap_x0; x <= c->gap_x1; x++)
             if (solid(x, 1)) return lbmx_fail(LBMX_ERR_ARG, "impedance opening under a solid node");
         if (!ade_stable(c->imp))

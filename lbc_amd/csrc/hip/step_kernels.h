@@ -195,10 +195,11 @@ __device__ __forceinline__ void collide_f32(float g[Q], const MomentsF &m, float
 // ---- FP64 arithmetic on FP32-stored deviations (LBMX_F32_WIDE) ---------------
 // Same storage and the same formulas as above, evaluated in FP64 and rounded to
 // FP32 once per stored value.  MI355X's FP64 vector rate equals its non-packed
-// FP32 rate, so in this memory-bound step the wider arithmetic costs no time,
-// while it removes the FP32 rounding of the equilibrium, whose error (~1e-8 of
-// g_eq per step, correlated from step to step) otherwise accumulates into the
-// slowly varying non-equilibrium part.
+// FP32 rate, so where the step is bound by memory the wider arithmetic costs no
+// time, while it removes the FP32 rounding of the equilibrium, whose error (~1e-8
+// of g_eq per step, correlated from step to step) otherwise accumulates into the
+// slowly varying non-equilibrium part.  Where several steps share one pass the
+// instruction count binds, hence the paired, fused form of collide_wide.
 struct MomentsW {
     double dr, ux, uy;
 };
@@ -222,53 +223,55 @@ __device__ __forceinline__ MomentsW moments_wide(const float g[Q], double rho0, 
     return m;
 }
 
+// Opposite directions k and k' = opp(k) see c.u with opposite signs, so their
+// equilibria are A +- B with A = w (dr + rho (4.5 (c.u)^2 - 1.5 u.u)) and
+// B = 3 w rho (c.u), and their Guo sources 3 c.F + 9 (c.u)(c.F) - 3 u.F are S +- 3 c.F
+// with S = 9 (c.u)(c.F) - 3 u.F: one evaluation serves the pair.  Products and
+// sums are fused where they meet (fma), which the FP64 reference form does not
+// do; here every value is rounded to FP32 afterwards, far above either rounding.
 template <int K>
-__device__ __forceinline__ float collide_wide_one(float g, const MomentsW &m, double rho,
+__device__ __forceinline__ void collide_wide_pair(float g[Q], const MomentsW &m, double rho,
                                                   double usq15, double om, bool force, double Fx,
                                                   double Fy, double uF3, double gf)
 {
-    constexpr int cx = D2Q9::cx[K], cy = D2Q9::cy[K];
-    const double w = D2Q9::w[K];
-    double cu = 0.0, geq;
-    if constexpr (K == 0) {
-        geq = w * (m.dr - rho * usq15);
-    } else {
-        cu = cdot<cx, cy>(m.ux, m.uy);
-        geq = w * (m.dr + rho * (cu * (3.0 + 4.5 * cu) - usq15));
-    }
-    const double gk = (double)g;
-    double r = gk - om * (gk - geq);
+    constexpr int cx = D2Q9::cx[K], cy = D2Q9::cy[K], K2 = D2Q9::opp[K];
+    constexpr double w = D2Q9::w[K];
+    const double cu = cdot<cx, cy>(m.ux, m.uy);
+    const double A = w * fma(rho, fma(4.5 * cu, cu, -usq15), m.dr);
+    const double B = (3.0 * w) * (rho * cu);
+    const double ga = (double)g[K], gb = (double)g[K2];
+    double ra = fma(-om, ga - (A + B), ga);
+    double rb = fma(-om, gb - (A - B), gb);
     if (force) {
-        double s;
-        if constexpr (K == 0) {
-            s = -uF3;
-        } else {
-            const double cF = cdot<cx, cy>(Fx, Fy);
-            s = 3.0 * cF + 9.0 * cu * cF - uF3;
-        }
-        r = r + gf * w * s;
+        const double cF = cdot<cx, cy>(Fx, Fy);
+        const double S = fma(9.0 * cu, cF, -uF3);
+        const double F3 = 3.0 * cF;
+        const double gw = gf * w;
+        ra = fma(gw, S + F3, ra);
+        rb = fma(gw, S - F3, rb);
     }
-    return (float)r;
-}
-
-template <int... Ks>
-__device__ __forceinline__ void collide_wide_all(float g[Q], const MomentsW &m, double rho,
-                                                 double usq15, double om, bool force, double Fx,
-                                                 double Fy, double uF3, double gf,
-                                                 std::integer_sequence<int, Ks...>)
-{
-    ((g[Ks] = collide_wide_one<Ks>(g[Ks], m, rho, usq15, om, force, Fx, Fy, uF3, gf)), ...);
+    g[K] = (float)ra;
+    g[K2] = (float)rb;
 }
 
 __device__ __forceinline__ void collide_wide(float g[Q], const MomentsW &m, double rho0, double om,
                                              bool force, double Fx, double Fy)
 {
     const double rho = rho0 + m.dr;
-    const double usq15 = 1.5 * (m.ux * m.ux + m.uy * m.uy);
-    const double uF3 = 3.0 * (m.ux * Fx + m.uy * Fy);
-    const double gf = 1.0 - 0.5 * om;
-    collide_wide_all(g, m, rho, usq15, om, force, Fx, Fy, uF3, gf,
-                     std::make_integer_sequence<int, Q>{});
+    const double usq15 = 1.5 * fma(m.ux, m.ux, m.uy * m.uy);
+    const double uF3 = 3.0 * fma(m.ux, Fx, m.uy * Fy);
+    const double gf = fma(-0.5, om, 1.0);
+    {
+        constexpr double w0 = D2Q9::w[0];
+        const double g0 = (double)g[0];
+        double r = fma(-om, g0 - w0 * fma(rho, -usq15, m.dr), g0);
+        if (force) r = fma(gf * w0, -uF3, r);
+        g[0] = (float)r;
+    }
+    collide_wide_pair<1>(g, m, rho, usq15, om, force, Fx, Fy, uF3, gf);   // (1, 3)
+    collide_wide_pair<2>(g, m, rho, usq15, om, force, Fx, Fy, uF3, gf);   // (2, 4)
+    collide_wide_pair<5>(g, m, rho, usq15, om, force, Fx, Fy, uF3, gf);   // (5, 7)
+    collide_wide_pair<8>(g, m, rho, usq15, om, force, Fx, Fy, uF3, gf);   // (8, 6)
 }
 
 // Lagged moments in the reference's FP64 form from either FP32 arithmetic.
