@@ -482,7 +482,8 @@ static int commit(lbmx_batch *b)
         // core tiles (kPairTX x pair_ty nodes), then band tiles (kBandTile nodes)
         for (int slot : b->order) {
             const BatchSlot &S = b->slots[slot];
-            const int nt = core_tiles(S.c.Nx, S.c.Ny, pair_ty((int)b->esize));
+            const int nt = core_tiles(S.c.Nx, S.c.Ny, pair_ty((int)b->esize),
+                                      wide_core(S.d.flags, S.c.Nx, S.c.Ny));
             for (int t = 0; t < nt; t++) T.push_back(Tile{slot, t});
         }
     } else {
@@ -496,7 +497,7 @@ static int commit(lbmx_batch *b)
     for (int slot : b->order) {
         const BatchSlot &S = b->slots[slot];
         if (b->pairs) {
-            const int nb = band_nodes(S.c.Nx, S.c.Ny);
+            const int nb = band_nodes(S.c.Nx, S.c.Ny, wide_core(S.d.flags, S.c.Nx, S.c.Ny));
             for (int b0 = 0; b0 < nb; b0 += kBandTile) T.push_back(Tile{slot, b0});
         } else {
             const int nf = frame_nodes(S.c.Nx, S.c.Ny);

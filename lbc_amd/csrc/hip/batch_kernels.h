@@ -362,33 +362,53 @@ __device__ __forceinline__ void batch_update(const BatchLat &L, int s, int x, in
 // takes its inner neighbour's populations; no other closure touches these rows.
 // Every lane stores all nine slots (a solid node stores zeros: its slots are
 // never read), so a wavefront writes whole cache lines.
-template <typename T, bool W, class SV, class OV, class DV>
-__device__ __forceinline__ void batch_row_node(const BatchLat &L, const SV &S, const OV &O,
-                                               const DV &D, int s, int x, int y, bool last)
+// The general gather of a row node: wrap, bounce-back, stale entries, and a
+// convective end node taking its inner neighbour's populations.  A solid node
+// gets zeros (and NaN completion moments) and returns false.
+template <typename T, class SV, class OV>
+__device__ __forceinline__ bool edge_gather(const BatchLat &L, const SV &S, const OV &O, int x,
+                                            int y, bool last, bool own, T f[Q])
+{
+    const int Nx = L.Nx;
+    const unsigned F = L.flags;
+    if (b_solid(L, x, y)) {
+        if (own && last && (F & BF_MOMENTS)) {
+            const int n = y * Nx + x;
+            gbl(L.out[0])[n] = NAN;
+            gbl(L.out[1])[n] = NAN;
+            gbl(L.out[2])[n] = NAN;
+        }
+#pragma unroll
+        for (int k = 0; k < Q; k++) f[k] = (T)0;
+        return false;
+    }
+    int gx = x;
+    if ((F & BF_CONVECTIVE) && (x == 0 || x == Nx - 1)) gx = (x == 0) ? 1 : Nx - 2;
+    b_gather<T>(L, S, O, gx, y, f);
+    return true;
+}
+
+template <typename T, bool W, class SV, class OV>
+__device__ __forceinline__ void row_node_update(const BatchLat &L, const SV &S, const OV &O, int s,
+                                                int x, int y, bool last, bool own, T f[Q])
 {
     const int Nx = L.Nx, Ny = L.Ny;
-    const int n = y * Nx + x;
-    const unsigned F = L.flags;
-    T f[Q];
+    bool fluid = true;
     if ((unsigned)(x - 2) < (unsigned)(Nx - 4) && (unsigned)(y - 2) < (unsigned)(Ny - 4)) {
 #pragma unroll
         for (int k = 0; k < Q; k++) f[k] = S.at(k, x - D2Q9::cx[k], y - D2Q9::cy[k]);
     } else {
-        if (b_solid(L, x, y)) {
-            if (last && (F & BF_MOMENTS)) {
-                gbl(L.out[0])[n] = NAN;
-                gbl(L.out[1])[n] = NAN;
-                gbl(L.out[2])[n] = NAN;
-            }
-#pragma unroll
-            for (int k = 0; k < Q; k++) D.put(k, x, y, (T)0);
-            return;
-        }
-        int gx = x;
-        if ((F & BF_CONVECTIVE) && (x == 0 || x == Nx - 1)) gx = (x == 0) ? 1 : Nx - 2;
-        b_gather<T>(L, S, O, gx, y, f);
+        fluid = edge_gather<T>(L, S, O, x, y, last, own, f);
     }
-    batch_update<T, W>(L, s, x, y, n, f, last);
+    if (fluid) batch_update<T, W>(L, s, x, y, y * Nx + x, f, last, own);
+}
+
+template <typename T, bool W, class SV, class OV, class DV>
+__device__ __forceinline__ void batch_row_node(const BatchLat &L, const SV &S, const OV &O,
+                                               const DV &D, int s, int x, int y, bool last)
+{
+    T f[Q];
+    row_node_update<T, W>(L, S, O, s, x, y, last, true, f);
 #pragma unroll
     for (int k = 0; k < Q; k++) D.put(k, x, y, f[k]);
 }
@@ -581,14 +601,18 @@ __global__ void __launch_bounds__(kFrameThreads) k_batch_frame(
 // ---- two steps per pass (pair mode) -------------------------------------------------
 // A lattice's nodes split into the core -- 3 <= x <= Nx-4 and 3 <= y <= Ny-4, where
 // a node and every source of its gather obey the plain rule -- and the band, the
-// nodes within 2 of an edge, where every boundary rule and closure lives.  With
-// the state f*_s in buf[X], X = (s / 2) & 1 (pairs start at even s), and Y = 1 - X:
+// nodes within 2 of an edge, where every boundary rule and closure lives.  A wide
+// core (wide_core) also takes the side columns of rows 3 .. Ny-4, whose nodes
+// then run the general gather inside the core tiles, and leaves the band rows
+// 0..2 and Ny-3..Ny-1.  With the state f*_s in buf[X], X = (s / 2) & 1 (pairs
+// start at even s), and Y = 1 - X:
 //   k_pair_band<0>  step s+1 of the band nodes: sources buf[X], stale entries
 //                   (f*_{s-1}) from the frame buffer's own slots, stores to the
 //                   frame buffer;
-//   k_pair_core     step s+1 of a core tile and a one-node ring in LDS (the ring
-//                   nodes are plain), the tile's nodes within 3 of an edge also to
-//                   the frame buffer, then step s+2 of the tile, stored to buf[Y];
+//   k_pair_core     step s+1 of a core tile and a one-node ring in LDS (each
+//                   neighbour recomputes the ring), the tile's nodes next to the
+//                   band also to the frame buffer, then step s+2 of the tile,
+//                   stored to buf[Y];
 //   k_pair_band<1>  step s+2 of the band nodes: sources from the frame buffer
 //                   (band nodes and the core's outer ring), stale entries (f*_s)
 //                   from buf[X]'s own slots, stores to buf[Y].
@@ -610,20 +634,36 @@ constexpr int kBandTile = 256;
 
 __host__ __device__ __forceinline__ int core_w(int Nx) { return Nx > 6 ? Nx - 6 : 0; }
 __host__ __device__ __forceinline__ int core_h(int Ny) { return Ny > 6 ? Ny - 6 : 0; }
-__host__ __device__ __forceinline__ int band_nodes(int Nx, int Ny)
+// A lattice whose side columns meet no stale entry in rows 3 .. Ny-4 -- each side
+// wraps, is a wall or is a convective end -- has a wide core: its core tiles span
+// every column (the side nodes take the general gather, from LDS in the second
+// step), and the band is rows 0..2 and Ny-3..Ny-1 only.  The three-node side
+// columns of a narrow core cost a cache line per plane and row each way.
+// A convective end node gathers around its inner neighbour, so the tile holding
+// column Nx-1 must also hold Nx-3 in its ring: not so when that tile is one
+// column wide (Nx = 64 t + 1), which keeps a narrow core.
+__host__ __device__ __forceinline__ bool wide_core(unsigned F, int Nx, int Ny)
 {
-    return Nx * Ny - core_w(Nx) * core_h(Ny);
+    if (frame_is_full(Nx, Ny)) return false;
+    if ((F & BF_CONVECTIVE) && Nx % kPairTX == 1) return false;
+    return (F & (BF_PER_X | BF_WALL_LEFT | BF_CONVECTIVE)) &&
+           (F & (BF_PER_X | BF_WALL_RIGHT | BF_CONVECTIVE));
 }
-// Core tiles cover columns [64 t, 64 t + 64) (cut to the core), so that a
-// tile's rows start on a 128-byte line when Nx is a multiple of 32, and rows
-// [3 + ty j, 3 + ty (j + 1)).
-__host__ __device__ __forceinline__ int core_tiles_x(int Nx)
+__host__ __device__ __forceinline__ int band_nodes(int Nx, int Ny, bool wide)
 {
+    return wide ? 6 * Nx : Nx * Ny - core_w(Nx) * core_h(Ny);
+}
+// Core tiles cover columns [64 t, 64 t + 64) (cut to the core, or to the lattice
+// when it is wide), so that a tile's rows start on a 128-byte line when Nx is a
+// multiple of 32, and rows [3 + ty j, 3 + ty (j + 1)).
+__host__ __device__ __forceinline__ int core_tiles_x(int Nx, bool wide)
+{
+    if (wide) return (Nx + kPairTX - 1) / kPairTX;
     return core_w(Nx) ? (Nx - 4 + kPairTX - 1) / kPairTX : 0;
 }
-__host__ __device__ __forceinline__ int core_tiles(int Nx, int Ny, int ty)
+__host__ __device__ __forceinline__ int core_tiles(int Nx, int Ny, int ty, bool wide)
 {
-    return core_tiles_x(Nx) * ((core_h(Ny) + ty - 1) / ty);
+    return core_tiles_x(Nx, wide) * ((core_h(Ny) + ty - 1) / ty);
 }
 
 // Workgroup -> item of a table of n items: XCD i (workgroups with blockIdx % 8 ==
@@ -636,30 +676,90 @@ __device__ __forceinline__ int xcd_item()
 }
 
 // Band node m (row-major over the band) -> (x, y).
-__device__ __forceinline__ void band_node(int Nx, int Ny, int m, int &x, int &y)
+__device__ __forceinline__ void band_node(int Nx, int Ny, bool wide, int m, int &x, int &y)
 {
     const int ch = core_h(Ny);
-    if (core_w(Nx) == 0 || ch == 0) {
+    if (!wide && (core_w(Nx) == 0 || ch == 0)) {
         y = m / Nx;
         x = m - y * Nx;
         return;
     }
+    const int side = wide ? 0 : 6 * ch;   // side-column nodes of a narrow core
     if (m < 3 * Nx) {
         y = m / Nx;
         x = m - y * Nx;
-    } else if (m < 3 * Nx + 6 * ch) {
+    } else if (m < 3 * Nx + side) {
         const int q = m - 3 * Nx, r = q / 6, c = q - 6 * r;
         y = 3 + r;
         x = c < 3 ? c : Nx - 6 + c;
     } else {
-        const int q = m - 3 * Nx - 6 * ch, r = q / Nx;
+        const int q = m - 3 * Nx - side, r = q / Nx;
         y = Ny - 3 + r;
         x = q - r * Nx;
     }
 }
 
+// A core tile's step-s+1 values in LDS: column lx = x - x0 + 1 (the ring's columns
+// of a periodic lattice hold the wrapped nodes), row ly = y - y0 + 1.
+template <typename T, int RX, int RN>
+struct TileV {
+    const T *b;
+    int x0, y0, Nx;
+    __device__ __forceinline__ T at(int k, int x, int y) const
+    {
+        int lx = x - x0 + 1;
+        if (lx < 0)
+            lx += Nx;
+        else if (lx >= RX)
+            lx -= Nx;
+        return b[k * RN + (y - y0 + 1) * RX + lx];
+    }
+};
+
+// The gather of a node in rows 2 .. Ny-3 of a wide core: b_gather's rules where no
+// source can be out of range or in a solid row -- the x axis wraps, or the side
+// is a wall (bounce-back from the node's own slots), or a convective end node
+// gathers around its inner neighbour; on a plain node (2 <= x <= Nx-3) it is the
+// plain gather.  A wall node gets zeros (and NaN completion moments) and returns
+// false.
+template <typename T, class SV>
+__device__ __forceinline__ bool side_gather(const BatchLat &L, const SV &S, int x, int y, bool last,
+                                            bool own, T f[Q])
+{
+    const int Nx = L.Nx;
+    const unsigned F = L.flags;
+    const bool wl = F & BF_WALL_LEFT, wr = F & BF_WALL_RIGHT;
+    if ((wl && x == 0) || (wr && x == Nx - 1)) {
+        if (own && last && (F & BF_MOMENTS)) {
+            const int n = y * Nx + x;
+            gbl(L.out[0])[n] = NAN;
+            gbl(L.out[1])[n] = NAN;
+            gbl(L.out[2])[n] = NAN;
+        }
+#pragma unroll
+        for (int k = 0; k < Q; k++) f[k] = (T)0;
+        return false;
+    }
+    int gx = x;
+    if ((F & BF_CONVECTIVE) && (x == 0 || x == Nx - 1)) gx = (x == 0) ? 1 : Nx - 2;
+    f[0] = S.at(0, gx, y);
+#pragma unroll
+    for (int k = 1; k < Q; k++) {
+        int xs = gx - D2Q9::cx[k];
+        if (xs < 0)
+            xs += Nx;
+        else if (xs >= Nx)
+            xs -= Nx;
+        const bool bb = (wl && xs == 0) || (wr && xs == Nx - 1);
+        f[k] = bb ? S.at(D2Q9::opp[k], gx, y) : S.at(k, xs, y - D2Q9::cy[k]);
+    }
+    return true;
+}
+
+// Six waves per SIMD is what the tile's LDS allows (three FP32 workgroups per CU);
+// the hint keeps the side-node gathers from costing a wave (at most 80 VGPRs).
 template <typename T, bool W, int TY = PairShape<T>::TY, int NT = PairShape<T>::threads>
-__global__ void __launch_bounds__(NT) k_pair_core(const BatchLat *__restrict__ lats,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) k_pair_core(const BatchLat *__restrict__ lats,
                                                            const Tile *__restrict__ tiles,
                                                            const BatchCtl *__restrict__ ctl,
                                                            int local)
@@ -675,31 +775,49 @@ __global__ void __launch_bounds__(NT) k_pair_core(const BatchLat *__restrict__ l
     if (s < 0 || s >= L.steps) return;
     const bool single = (s + 1 == L.steps);
     const int Nx = L.Nx, Ny = L.Ny;
-    const int ntx = core_tiles_x(Nx);
+    const bool wide = wide_core(L.flags, Nx, Ny);
+    const bool px = L.flags & BF_PER_X;
+    const int ntx = core_tiles_x(Nx, wide);
     const int x0 = (tl.start % ntx) * kPairTX, y0 = 3 + (tl.start / ntx) * TY;
+    // the tile's own nodes: columns [xa, xb], rows [y0, min(y0 + TY, Ny - 3))
+    const int xa = wide ? x0 : max(x0, 3);
+    const int xb = wide ? min(x0 + kPairTX, Nx) - 1 : min(x0 + kPairTX - 1, Nx - 4);
     const int X = (int)((s >> 1) & 1);
     const FullV<T> S = full_view<T>(L, X), D = full_view<T>(L, X ^ 1);
     const FrameV<T> R = frame_view<T>(L);
+    // a tile reaching a side node (x < 2 or x > Nx-3; its ring spans x0-1 .. x0+64)
+    // gathers every node by the side rule, which equals the plain gather on plain
+    // nodes: the choice is the workgroup's, so no wavefront runs both
+    const bool edge = wide && (x0 < 2 || x0 + kPairTX >= Nx - 2);
 
-    // step s+1: the tile and its one-node ring (plain nodes: 2 <= x <= Nx-3)
+    // step s+1: the tile and its one-node ring (columns xa-1 .. xb+1, wrapped on a
+    // periodic lattice; a narrow core's ring is plain, 2 <= x <= Nx-3)
     for (int r = threadIdx.x; r < RN; r += NT) {
         const int lx = r % RX, ly = r / RX;
-        const int x = x0 - 1 + lx, y = y0 - 1 + ly;
-        // only plain nodes (2 <= x <= Nx-3) are read by the tile's core nodes
-        if (x < 2 || x > Nx - 3 || y > Ny - 3) continue;
-        const bool own = lx >= 1 && lx <= kPairTX && ly >= 1 && ly <= TY && x >= 3 &&
-                         x <= Nx - 4 && y <= Ny - 4;
+        const int xu = x0 - 1 + lx, y = y0 - 1 + ly;
+        if (y > Ny - 3 || xu < xa - 1 || xu > xb + 1) continue;
+        int x = xu;
+        if (x < 0 || x >= Nx) {
+            if (!px) continue;
+            x += x < 0 ? Nx : -Nx;
+        }
+        const bool own = xu >= xa && xu <= xb && ly >= 1 && ly <= TY && y <= Ny - 4;
         if (single && !own) continue;
         T p[Q];
+        bool fluid = true;
+        if (!edge) {
 #pragma unroll
-        for (int k = 0; k < Q; k++) p[k] = S.at(k, x - D2Q9::cx[k], y - D2Q9::cy[k]);
-        batch_update<T, W>(L, (int)s, x, y, y * Nx + x, p, single, own);
+            for (int k = 0; k < Q; k++) p[k] = S.at(k, x - D2Q9::cx[k], y - D2Q9::cy[k]);
+        } else {
+            fluid = side_gather<T>(L, S, x, y, single, own, p);
+        }
+        if (fluid) batch_update<T, W>(L, (int)s, x, y, y * Nx + x, p, single, own);
         if (single) {
 #pragma unroll
             for (int k = 0; k < Q; k++) D.put(k, x, y, p[k]);
             continue;
         }
-        if (own && (x == 3 || y == 3 || x == Nx - 4 || y == Ny - 4)) {
+        if (own && (y == 3 || y == Ny - 4 || (!wide && (x == 3 || x == Nx - 4)))) {
 #pragma unroll
             for (int k = 0; k < Q; k++) R.put(k, x, y, p[k]);
         }
@@ -711,15 +829,21 @@ __global__ void __launch_bounds__(NT) k_pair_core(const BatchLat *__restrict__ l
 
     // step s+2: the tile, from LDS
     const bool last = (s + 2 == L.steps);
+    const TileV<T, RX, RN> V{buf, x0, y0, Nx};
     for (int q = threadIdx.x; q < kPairTX * TY; q += NT) {
         const int tx = q % kPairTX, ty = q / kPairTX;
         const int x = x0 + tx, y = y0 + ty;
-        if (x < 3 || x > Nx - 4 || y > Ny - 4) continue;
+        if (x < xa || x > xb || y > Ny - 4) continue;
         T p[Q];
+        bool fluid = true;
+        if (!edge) {
 #pragma unroll
-        for (int k = 0; k < Q; k++)
-            p[k] = buf[k * RN + (ty + 1 - D2Q9::cy[k]) * RX + (tx + 1 - D2Q9::cx[k])];
-        batch_update<T, W>(L, (int)s + 1, x, y, y * Nx + x, p, last);
+            for (int k = 0; k < Q; k++)
+                p[k] = buf[k * RN + (ty + 1 - D2Q9::cy[k]) * RX + (tx + 1 - D2Q9::cx[k])];
+        } else {
+            fluid = side_gather<T>(L, V, x, y, last, true, p);
+        }
+        if (fluid) batch_update<T, W>(L, (int)s + 1, x, y, y * Nx + x, p, last);
 #pragma unroll
         for (int k = 0; k < Q; k++) D.put(k, x, y, p[k]);
     }
@@ -752,9 +876,10 @@ __global__ void __launch_bounds__(kBandTile) k_pair_band(const BatchLat *__restr
     const bool single = (s + 1 == L.steps);
     if (PHASE == 1 && single) return;
     const int m = tl.start + (int)threadIdx.x;
-    if (m >= band_nodes(L.Nx, L.Ny)) return;
+    const bool wide = wide_core(L.flags, L.Nx, L.Ny);
+    if (m >= band_nodes(L.Nx, L.Ny, wide)) return;
     int x, y;
-    band_node(L.Nx, L.Ny, m, x, y);
+    band_node(L.Nx, L.Ny, wide, m, x, y);
     const int X = (int)((s >> 1) & 1);
     const FullV<T> A = full_view<T>(L, X), B = full_view<T>(L, X ^ 1);
     const FrameV<T> R = frame_view<T>(L);

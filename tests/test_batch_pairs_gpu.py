@@ -6,6 +6,9 @@ Pair mode must give, bit for bit, the moments and probe histories of one step pe
 launch (which the single-step tests pin against the oracle in FP64):
 * every case family and the liner duct, odd and even lengths, lattices too small
   to have a core (all band) or with a core of one column, periodic and walled;
+  wide cores (side columns inside the core tiles: periodic, walled and
+  convective sides) narrower than a tile, 65 and 66 columns wide, the smallest
+  (9 x 9), and narrow cores (a side with stale entries: open ends, one wall);
 * lattices joining the batch at later epochs, several graph chunk lengths;
 * the FP64 liner duct against the oracle's liner driver directly;
 * the divergence guard: a NaN lattice trips alone.
@@ -39,6 +42,20 @@ def zoo():
                      sponge=24, probes=[(96, 20), (300, 20)]),
         B.liner_duct(257, 27, 300, imp, amp=5e-4, x0=40.0, halfwidth=6.0, liner=(90, 170),
                      sponge=12, probes=[(3, 3), (200, 13)]),
+        # open, non-periodic ends (stale entries on both sides): a narrow core
+        B.Case(96, 40, 201, tau=0.8, walls=B.WALL_BOTTOM | B.WALL_TOP, Fx=1e-5, kind="open"),
+        # a wall on the left only, stale entries on the right: a narrow core
+        B.Case(70, 33, 150, tau=0.7, walls=B.WALL_BOTTOM | B.WALL_LEFT,
+               top_velocity=(0.02, 0.0), kind="leftwall"),
+        # wide cores narrower than a tile, one column past a tile, a tile plus the ring
+        B.couette(40, 30, 0.7, 120),
+        B.poiseuille(65, 20, 0.9, 99, Fx=1e-5),
+        B.taylor_green(66, 12, 90),
+        B.cavity(9, 9, 10.0, 40),             # the smallest wide core
+        # convective ends, last tile two columns wide (the 257-column duct above,
+        # one column wide, keeps a narrow core)
+        B.liner_duct(258, 27, 300, imp, amp=5e-4, x0=40.0, halfwidth=6.0, liner=(90, 170),
+                     sponge=12, probes=[(3, 3), (257, 13)]),
     ]
 
 
@@ -85,10 +102,26 @@ def test_pairs_equal_single_steps(prec):
 @pytest.mark.parametrize("prec", [native.F64, native.F32_WIDE], ids=["f64", "f32wide"])
 def test_pairs_staggered_admission_and_chunks(prec):
     cs = zoo()
-    sched = [(0, [0, 2, 4, 8]), (6, [1, 5]), (40, [3, 6, 7, 9])]
+    sched = [(0, [0, 2, 4, 8, 10, 14]), (6, [1, 5, 11, 13]), (40, [3, 6, 7, 9, 12, 15])]
     ref = run(cs, prec, False, chunk=32, schedule=sched)
     for chunk in (2, 6, 32):
         assert_same(run(cs, prec, True, chunk=chunk, schedule=sched), ref)
+
+
+def test_pairs_fp64_equal_oracle():
+    """Open ends, a single side wall, the small wide cores and the wide duct
+    against the oracle."""
+    cs = zoo()[10:]
+    got = run(cs, native.F64, True)
+    for i, c in enumerate(cs):
+        if c.kind == "liner":
+            probe, rho_ref = TB.oracle_liner(c, c.steps)
+            assert TB.same(got[i][0][0], rho_ref), c.kind
+            assert np.array_equal(got[i][1][:c.steps, 1], probe), c.kind
+            continue
+        ref = TB.oracle_moments(c, c.steps)
+        for x, y in zip(got[i][0], ref):
+            assert TB.same(x, y), c.kind
 
 
 def test_pairs_liner_fp64_equals_oracle():
