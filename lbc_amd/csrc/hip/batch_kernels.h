@@ -750,8 +750,9 @@ __device__ __forceinline__ bool side_gather(const BatchLat &L, const SV &S, int 
             xs += Nx;
         else if (xs >= Nx)
             xs -= Nx;
+        // bounce-back reads the node's own opposite slot: one load either way
         const bool bb = (wl && xs == 0) || (wr && xs == Nx - 1);
-        f[k] = bb ? S.at(D2Q9::opp[k], gx, y) : S.at(k, xs, y - D2Q9::cy[k]);
+        f[k] = S.at(bb ? D2Q9::opp[k] : k, bb ? gx : xs, bb ? y : y - D2Q9::cy[k]);
     }
     return true;
 }
@@ -785,27 +786,25 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) k_
     const int X = (int)((s >> 1) & 1);
     const FullV<T> S = full_view<T>(L, X), D = full_view<T>(L, X ^ 1);
     const FrameV<T> R = frame_view<T>(L);
-    // a tile reaching a side node (x < 2 or x > Nx-3; its ring spans x0-1 .. x0+64)
-    // gathers every node by the side rule, which equals the plain gather on plain
-    // nodes: the choice is the workgroup's, so no wavefront runs both
+    // A tile whose ring reaches a side node (x < 2 or x > Nx-3: the ring spans
+    // x0-1 .. x0+64) lists its side nodes first and its plain nodes after them;
+    // the first whole wavefronts take the side rule (which on a plain node is the
+    // plain gather), the rest the plain one, so no wavefront runs both branches
+    // and the item count, hence the passes per thread, is an interior tile's.
     const bool edge = wide && (x0 < 2 || x0 + kPairTX >= Nx - 2);
 
-    // step s+1: the tile and its one-node ring (columns xa-1 .. xb+1, wrapped on a
-    // periodic lattice; a narrow core's ring is plain, 2 <= x <= Nx-3)
-    for (int r = threadIdx.x; r < RN; r += NT) {
-        const int lx = r % RX, ly = r / RX;
-        const int xu = x0 - 1 + lx, y = y0 - 1 + ly;
-        if (y > Ny - 3 || xu < xa - 1 || xu > xb + 1) continue;
+    // step s+1 of the node in ring column xu, LDS row ly
+    auto step1 = [&](int xu, int ly, bool side_rule) {
+        const int y = y0 - 1 + ly;
+        if (y > Ny - 3) return;
         int x = xu;
-        if (x < 0 || x >= Nx) {
-            if (!px) continue;
-            x += x < 0 ? Nx : -Nx;
-        }
+        if (x < 0 || x >= Nx) x += x < 0 ? Nx : -Nx;   // (only on a periodic lattice)
         const bool own = xu >= xa && xu <= xb && ly >= 1 && ly <= TY && y <= Ny - 4;
-        if (single && !own) continue;
+        if (single && !own) return;
+        const int r = ly * RX + (xu - x0 + 1);
         T p[Q];
         bool fluid = true;
-        if (!edge) {
+        if (!side_rule) {
 #pragma unroll
             for (int k = 0; k < Q; k++) p[k] = S.at(k, x - D2Q9::cx[k], y - D2Q9::cy[k]);
         } else {
@@ -815,7 +814,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) k_
         if (single) {
 #pragma unroll
             for (int k = 0; k < Q; k++) D.put(k, x, y, p[k]);
-            continue;
+            return;
         }
         if (own && (y == 3 || y == Ny - 4 || (!wide && (x == 3 || x == Nx - 4)))) {
 #pragma unroll
@@ -823,20 +822,48 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) k_
         }
 #pragma unroll
         for (int k = 0; k < Q; k++) buf[k * RN + r] = p[k];
+    };
+
+    // step s+1: the tile and its one-node ring, columns xa-1 .. xb+1 (wrapped on a
+    // periodic lattice; a narrow core's ring is plain, 2 <= x <= Nx-3)
+    if (!edge) {
+        for (int r = threadIdx.x; r < RN; r += NT) {
+            const int lx = r % RX, xu = x0 - 1 + lx;
+            if (xu >= xa - 1 && xu <= xb + 1) step1(xu, r / RX, false);
+        }
+    } else {
+        // side columns [la, 1] and [Nx-2, rb], plain columns [pa, pb] between
+        const int la = px ? -1 : 0, rb = min(px ? Nx : Nx - 1, xb + 1);
+        const int nl = x0 == 0 ? 2 - la : 0, nr = xb + 1 >= Nx - 2 ? rb - (Nx - 2) + 1 : 0;
+        const int pa = max(x0 - 1, 2), pb = min(xb + 1, Nx - 3), pw = max(pb - pa + 1, 0);
+        const int ns = (nl + nr) * (TY + 2), n = ns + pw * (TY + 2);
+        const int nsr = min((ns + 63) & ~63, n);   // whole wavefronts take the side rule
+        for (int i = threadIdx.x; i < n; i += NT) {
+            int xu, ly;
+            if (i < ns) {
+                const int c = i % (nl + nr);
+                xu = c < nl ? la + c : Nx - 2 + (c - nl);
+                ly = i / (nl + nr);
+            } else {
+                const int j = i - ns;
+                xu = pa + j % pw;
+                ly = j / pw;
+            }
+            step1(xu, ly, i < nsr);
+        }
     }
     if (single) return;   // the whole workgroup: the lattice's last step was this one
     __syncthreads();
 
-    // step s+2: the tile, from LDS
+    // step s+2 of tile node x, row ty, from LDS
     const bool last = (s + 2 == L.steps);
     const TileV<T, RX, RN> V{buf, x0, y0, Nx};
-    for (int q = threadIdx.x; q < kPairTX * TY; q += NT) {
-        const int tx = q % kPairTX, ty = q / kPairTX;
-        const int x = x0 + tx, y = y0 + ty;
-        if (x < xa || x > xb || y > Ny - 4) continue;
+    auto step2 = [&](int x, int ty, bool side_rule) {
+        const int y = y0 + ty, tx = x - x0;
+        if (y > Ny - 4) return;
         T p[Q];
         bool fluid = true;
-        if (!edge) {
+        if (!side_rule) {
 #pragma unroll
             for (int k = 0; k < Q; k++)
                 p[k] = buf[k * RN + (ty + 1 - D2Q9::cy[k]) * RX + (tx + 1 - D2Q9::cx[k])];
@@ -846,6 +873,32 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) k_
         if (fluid) batch_update<T, W>(L, (int)s + 1, x, y, y * Nx + x, p, last);
 #pragma unroll
         for (int k = 0; k < Q; k++) D.put(k, x, y, p[k]);
+    };
+    if (!edge) {
+        for (int q = threadIdx.x; q < kPairTX * TY; q += NT) {
+            const int x = x0 + q % kPairTX;
+            if (x >= xa && x <= xb) step2(x, q / kPairTX, false);
+        }
+    } else {
+        // side columns [xa, 1] and [max(Nx-2, xa), xb], plain columns between
+        const int nl = xa <= 1 ? 2 - xa : 0;
+        const int ra = max(Nx - 2, xa), nr = xb >= ra ? xb - ra + 1 : 0;
+        const int qa = max(xa, 2), qw = max(min(xb, Nx - 3) - qa + 1, 0);
+        const int ns = (nl + nr) * TY, n = ns + qw * TY;
+        const int nsr = min((ns + 63) & ~63, n);
+        for (int i = threadIdx.x; i < n; i += NT) {
+            int x, ty;
+            if (i < ns) {
+                const int c = i % (nl + nr);
+                x = c < nl ? xa + c : ra + (c - nl);
+                ty = i / (nl + nr);
+            } else {
+                const int j = i - ns;
+                x = qa + j % qw;
+                ty = j / qw;
+            }
+            step2(x, ty, i < nsr);
+        }
     }
 }
 
