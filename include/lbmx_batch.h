@@ -166,9 +166,11 @@ LBMX_API int lbmx_batch_get_tuning(lbmx_batch *b, lbmx_tuning *t);
 LBMX_API int lbmx_batch_set_chunk(lbmx_batch *b, int steps);
 
 /* Pair mode (on != 0): two time steps per pass over each lattice.  A lattice's
- * core (nodes at least 3 from every edge) takes both steps in one launch, its
- * intermediate step kept in LDS; the nodes nearer an edge, where every boundary
- * rule lives, take one step per launch through a small frame buffer.  Results
+ * core (nodes at least 3 from the top and bottom rows, and from the side columns
+ * unless every side node wraps, is a wall or is a convective end) takes both
+ * steps in one launch, its intermediate step kept in LDS; the other nodes, where
+ * the remaining boundary rules live, take one step per launch through a small
+ * frame buffer.  Results
  * are bit-identical to one step per launch; HBM traffic per step about halves.
  * The batch then advances in steps of two (lbmx_batch_run, lbmx_batch_time and
  * the chunk must be even; a lattice with an odd `steps` takes a single step at
