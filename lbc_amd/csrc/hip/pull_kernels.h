@@ -455,6 +455,138 @@ __global__ void __launch_bounds__(LW * D) k_stream(const T *__restrict__ S, T *_
     }
 }
 
+// ---- D steps per launch in one wavefront's registers --------------------------
+// k_wave: each wavefront owns a strip of 64 columns (TX = 64 - 2(D-1) of them
+// written) and a segment of rows, and walks down the segment computing, per
+// row, every time level in program order: level 1 gathers row t from S in
+// global memory, level l (2..D) row t - (l-1) from level l-1's last three rows
+// held in registers -- a lane's x - 1 and x + 1 neighbours come from the lanes
+// beside it (DPP wave shifts), rows from the registers -- and level D stores its
+// row.  Nothing passes through LDS and no barrier is needed; every row of every
+// level is computed (rows beyond a segment are the periodic lattice's own rows,
+// needed as the levels' ring or computed and dropped), so all lanes run the same
+// instructions.  Level l is valid on lanes l-1 .. 64-l, whose sources are valid
+// lanes of level l-1.  Same arithmetic on the same values as D k_pull launches:
+// bit-identical.
+constexpr int kWaveWaves = 4;   // independent wavefronts per workgroup
+
+// lane i receives lane i-1 (wave_shr:1) or lane i+1 (wave_shl:1) of v; the edge
+// lane keeps its own value (it is outside the valid lanes of the next level)
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL,
+                                                      0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xf, 0xf,
+                                               false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+constexpr int kDppShr1 = 0x138, kDppShl1 = 0x130;
+
+// What a level keeps of its last rows for the next level: the newest row n's
+// populations {0, 1, 3} and {2, 5, 6}, and row n-1's {2, 5, 6}.
+template <typename T>
+struct WaveRows {
+    T c013[3], c256[3], p256[3];
+};
+
+// Gather of row q of level l from level l-1: row q+1 is `nw` (just computed), row q
+// and row q-1 are what `h` kept.  Then `h` moves on to row q+1.
+template <typename T>
+__device__ __forceinline__ void wave_gather(WaveRows<T> &h, const T nw[Q], T f[Q])
+{
+    f[0] = h.c013[0];
+    f[1] = dpp_shift<kDppShr1>(h.c013[1]);   // c = (+1, 0): from x-1
+    f[3] = dpp_shift<kDppShl1>(h.c013[2]);   // c = (-1, 0): from x+1
+    f[2] = h.p256[0];                        // c = (0, +1): from row q-1
+    f[5] = dpp_shift<kDppShr1>(h.p256[1]);   // c = (+1, +1)
+    f[6] = dpp_shift<kDppShl1>(h.p256[2]);   // c = (-1, +1)
+    f[4] = nw[4];                            // c = (0, -1): from row q+1
+    f[8] = dpp_shift<kDppShr1>(nw[8]);       // c = (+1, -1)
+    f[7] = dpp_shift<kDppShl1>(nw[7]);       // c = (-1, -1)
+    h.p256[0] = h.c256[0];
+    h.p256[1] = h.c256[1];
+    h.p256[2] = h.c256[2];
+    h.c013[0] = nw[0];
+    h.c013[1] = nw[1];
+    h.c013[2] = nw[3];
+    h.c256[0] = nw[2];
+    h.c256[1] = nw[5];
+    h.c256[2] = nw[6];
+}
+template <typename T, bool W, int D, int P>
+__global__ void __launch_bounds__(64 * kWaveWaves) k_wave(const T *__restrict__ S,
+                                                         T *__restrict__ Dst, StreamArgs a)
+{
+    constexpr int TX = 64 - 2 * (D - 1);
+    const Geo &g = a.g;
+    const int lane = threadIdx.x % 64;
+    // XCD-aware order: XCD i (workgroups with blockIdx % 8 == i) takes the i-th
+    // eighth of the strip-major work list
+    const int work = a.strips * a.segs;
+    const int per = (int)gridDim.x / 8;
+    const int item = (((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8) * kWaveWaves +
+                     __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    if (item >= work) return;   // whole wavefront (no barrier in this kernel)
+    const int y0 = (item / a.strips) * a.seg_rows;
+    const int y1 = min(y0 + a.seg_rows, g.Ny);
+    const int x0 = (item % a.strips) * TX;
+    const int x = ((x0 - (D - 1) + lane) % g.Nx + g.Nx) % g.Nx;
+    const unsigned bx = (unsigned)x * (unsigned)sizeof(T);
+    const unsigned bxm = (unsigned)(x == 0 ? g.Nx - 1 : x - 1) * (unsigned)sizeof(T);
+    const unsigned bxp = (unsigned)(x == g.Nx - 1 ? 0 : x + 1) * (unsigned)sizeof(T);
+    const int xo = x0 + lane - (D - 1);   // output column of this lane at level D
+    const bool out_lane = lane >= D - 1 && lane <= 64 - D && xo < g.Nx;
+    const int rbase = y0 - (D - 1);
+    const int iters = (y1 - y0) + 2 * (D - 1);
+    auto row = [&](int t) {   // periodic row of level-1 iteration t
+        int r = rbase + t;
+        while (r < 0) r += g.Ny;
+        while (r >= g.Ny) r -= g.Ny;
+        return r;
+    };
+
+    WaveRows<T> h[D - 1];
+    T pre[P][Q];
+#pragma unroll
+    for (int j = 0; j < P; j++) gather_row(S, g, row(j), bxm, bx, bxp, pre[j]);
+    for (int t0 = 0; t0 < iters; t0 += P) {
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            const int t = t0 + j;
+            T nw[Q], f[Q];
+#pragma unroll
+            for (int k = 0; k < Q; k++) nw[k] = pre[j][k];
+            gather_row(S, g, row(t + P), bxm, bx, bxp, pre[j]);
+            // level 1: row rbase + t
+            pull_collide<T, W>(nw, g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
+            // levels 2 .. D: row rbase + t - (l-1)
+#pragma unroll
+            for (int l = 2; l <= D; l++) {
+                wave_gather(h[l - 2], nw, f);
+                pull_collide<T, W>(f, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
+#pragma unroll
+                for (int k = 0; k < Q; k++) nw[k] = f[k];
+            }
+            const int r = rbase + t - (D - 1);
+            if (out_lane && r >= y0 && r < y1) {
+                char *rowp = reinterpret_cast<char *>(Dst + (size_t)r * g.pitch);
+                const unsigned bo = (unsigned)xo * (unsigned)sizeof(T);
+#pragma unroll
+                for (int k = 0; k < Q; k++)
+                    store_nt(reinterpret_cast<T *>(rowp + (size_t)k * g.plane * sizeof(T) + bo),
+                             nw[k]);
+            }
+        }
+    }
+}
+
 // f_k = gather(f*_{k-1}) (streaming only).
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather(const T *__restrict__ S, T *__restrict__ D, Geo g)
