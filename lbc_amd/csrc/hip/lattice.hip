@@ -254,6 +254,7 @@ extern "C" void lbmx_destroy(lbmx_lattice *L)
     hipFree(L->om_arr);
     hipFree(L->links);
     hipFree(L->partials);
+    lbmx_chan_free(L);
     hipFree(L->force_dev);
     if (L->frec) hipFree(L->frec);
     hipFree(L->bc_dev);
@@ -293,6 +294,8 @@ extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double 
     if (!L || !mask || !omega) return lbmx_fail(LBMX_ERR_ARG, "null argument");
     if (L->cfg.use_ibb && !phi) return lbmx_fail(LBMX_ERR_ARG, "use_ibb requires phi");
     if (int prc = lbmx_ensure_push(L)) return prc;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    lbmx_chan_free(L);
     const int Nx = L->Nx, Ny = L->Ny, pitch = L->pitch;
     const size_t nn = (size_t)Nx * Ny;
     static const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
@@ -331,6 +334,8 @@ extern "C" int lbmx_set_geometry(lbmx_lattice *L, const int *mask, const double 
         for (size_t n = 0; n < nn; n++) L->hsolid[n] = mask[n] ? 1 : 0;
     }
     L->tail_planned = 0;
+    L->nclasses = (int)cls_of.size();
+    L->hlut1 = lut[1];
     if (nsolid == 0 && cls_of.size() == 1) {
         L->geom = 0;
         L->om_uniform = lut[1];
@@ -443,8 +448,11 @@ extern "C" int lbmx_set_uniform(lbmx_lattice *L, double omega)
     hipFree(L->om_arr); L->om_arr = nullptr;
     hipFree(L->links); L->links = nullptr;
     hipFree(L->partials); L->partials = nullptr;
+    lbmx_chan_free(L);
     L->geom = 0;
     L->om_uniform = omega;
+    L->nclasses = 1;
+    L->hlut1 = omega;
     L->nsolid = 0;
     L->hsolid.clear();
     L->tail_planned = 0;
@@ -482,6 +490,7 @@ extern "C" int lbmx_set_boundaries(lbmx_lattice *L, const int *types, const int 
     L->types.assign(types, types + nbc);
     L->index.assign(index, index + nbc);
     L->vals_set.assign(nbc, 0);
+    lbmx_chan_free(L);   // eligibility, columns and work lists depend on the table
 
     std::vector<BcDesc> prog;
     std::vector<int> edge;
@@ -1607,7 +1616,12 @@ extern "C" int lbmx_set_launch(lbmx_lattice *L, int mode)
 extern "C" int lbmx_launch_depth(const lbmx_lattice *L)
 {
     if (!L) return lbmx_fail(LBMX_ERR_ARG, "null argument");
-    if (!pull_eligible(L)) return 1;
+    if (!pull_eligible(L)) {
+        lbmx_lattice *M = const_cast<lbmx_lattice *>(L);   // caches the eligibility check
+        const bool on = L->launch_mode == LBMX_LAUNCH_STREAM ||
+                        (L->launch_mode == LBMX_LAUNCH_AUTO && lbmx_chan_fits(L));
+        return on && lbmx_chan_eligible(M) ? lbmx_chan_depth() : 1;
+    }
     int d = 1, c = 0, r = 0;
     const int rc = lbmx_launch_plan(L->Nx, L->Ny, L->cfg.precision, L->launch_mode, &d, &c, &r);
     return rc ? rc : d;
@@ -1623,6 +1637,7 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
     if (nsteps <= 0) return LBMX_OK;
     const bool f64 = (L->esize == 8);
     const int64_t steps0 = L->steps;
+    const int mode = L->launch_mode;
     if (!L->pull && nsteps >= 2 && pull_eligible(L)) {
         int rc = f64 ? enter_pull<double>(L) : enter_pull<float>(L);
         if (rc) return rc;
@@ -1632,11 +1647,19 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
         if (rc) return rc;
     }
     int i = L->steps - steps0;   // 1 when entering the gather form took the first step
+    // bounded lattices (walls, bodies, regularized columns): several steps per
+    // launch when row streaming is on for them, the call's last 1..depth steps
+    // through the one-step path (which leaves both buffers as the reference's)
+    if (!L->pull && (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && lbmx_chan_fits(L))) &&
+        nsteps > lbmx_chan_depth() && lbmx_chan_eligible(L)) {
+        const int cd = lbmx_chan_depth();
+        for (const int n = (nsteps - 1) / cd; i < n * cd; i += cd)
+            if (int rc = lbmx_chan_run(L)) return rc;
+    }
     // tile depth: 3 steps per launch (the tiles serve the lattices too small for
     // row streaming, where three beat two in FP64 as in FP32: 256^2 3.69 vs 3.97
     // us per step, 512^2 4.66 vs 5.05); LBMX_TEMPORAL_DEPTH=2 or 3 overrides
     const int depth = tile_depth();
-    const int mode = L->launch_mode;
     if (L->pull && mode != LBMX_LAUNCH_SINGLE) {
         // multi-step launches while at least as many steps remain as they take; the
         // buffer a launch writes is the post-collision field of its last step, all a

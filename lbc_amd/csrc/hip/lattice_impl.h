@@ -10,6 +10,10 @@
 #include "step_kernels.h"
 #include "tdibc_kernels.h"
 
+namespace lbmx {
+struct ChanAux;
+}
+
 constexpr int kQ = 9;
 
 int lbmx_fail(int code, const char *fmt, ...);
@@ -87,6 +91,23 @@ struct lbmx_lattice {
     // gather form (pull_kernels.h): buffers hold post-collision populations;
     // pop[pstar] is the one the next step streams from
     int pull = 0, pstar = 0;
+    // bounded multi-step launch (chan_kernels.h, chan.hip): lattices with walls,
+    // bodies or regularized columns
+    int chan_ok = -1;              // lattice qualifies (-1: not checked since a change)
+    int chan_lo = -1, chan_hi = -1;   // program entries of the x = 0 / x = Nx-1 columns
+    int chan_aux_dirty = 1;          // chan_aux must be rewritten before the next launch
+    uint2 *chan_node = nullptr;      // per node: class | link bits << 8, first link
+    double *chan_lq = nullptr;       // per link: q or -1 (plain bounce-back)
+    double *chan_ldf = nullptr;      // per step of a launch and link: force term
+    double *chan_part = nullptr;     // per step and link block: force partials
+    lbmx::ChanAux *chan_aux = nullptr;   // the kernels' rare-path tables
+    unsigned *chan_done = nullptr;   // force kernel: blocks finished
+    int4 *chan_work = nullptr;       // work items of a launch: x0, y0, y1, bounded path
+    long long chan_work_key = -1;    // the launch shape chan_work was built for
+    int chan_nwork = 0, chan_nspecial = 0;
+    std::vector<unsigned> chan_plain_rows;   // per row: prefix counts of non-plain nodes
+    int nclasses = 0;                // fluid classes (geom 1)
+    double hlut1 = 1.0;              // omega of class 1
 
     // time-domain impedance boundary controller (lbmx_set_tdibc)
     struct {
@@ -130,3 +151,10 @@ int lbmx_materialize_other(lbmx_lattice *L);
 void lbmx_reset_run(lbmx_lattice *L);
 // Returns the lattice to the reference's two-buffer state if it runs in gather form.
 int lbmx_ensure_push(lbmx_lattice *L);
+
+// bounded multi-step launch (chan.hip)
+void lbmx_chan_free(lbmx_lattice *L);
+int lbmx_chan_eligible(lbmx_lattice *L);
+int lbmx_chan_fits(const lbmx_lattice *L);
+int lbmx_chan_run(lbmx_lattice *L);
+int lbmx_chan_depth();
