@@ -1337,14 +1337,32 @@ static long long stream_wgs_of(int Nx, int Ny, int depth)
 // per row; GLUPS 32 vs 16 rows: FP64 768^2 54.4 vs 66.5, 1024^2 75.4 vs 77.1,
 // 1280^2 (920 workgroups) 89.5 vs 85.0; FP32 1024^2 130.5 vs 140.9, 1280^2 167.2
 // vs 155.5; FP32 + FP64 collision 1024^2 87.1 vs 92.8, 1280^2 111.0 vs 97.5.
+//
+// A grid a little larger than the workgroups the CUs hold at once (1,024 at four
+// per CU on 256 CUs) runs its last partial round after the first: where 16-row
+// segments give up to 1.5 times that many, the segments are lengthened until the
+// grid fits one round (1024^2: 19 rows, 972 workgroups; GLUPS 16 vs 19 rows FP64
+// 74.4 vs 83.2, FP32 139.0 vs 146.8, FP32 + FP64 collision 81.5 vs 96.6).
+constexpr long long kStreamSlots = 1024;
 static int stream_rows_of(int Nx, int Ny, int depth)
 {
     const int rows = stream_rows_base(Nx, Ny, depth);
-    if (depth == 4 && rows == 32 && stream_wgs_of(Nx, Ny, depth) < 768) return 16;
+    if (depth == 4 && rows == 32 && stream_wgs_of(Nx, Ny, depth) < 768) {
+        const int tx = kStreamLW - 2 * (depth - 1);
+        const long long strips = (Nx + tx - 1) / tx;
+        const long long wgs = strips * ((Ny + 15) / 16);
+        if (wgs > kStreamSlots && wgs * 2 <= kStreamSlots * 3 && strips <= kStreamSlots) {
+            const long long segs = kStreamSlots / strips;
+            return (int)std::min<long long>(Ny, (Ny + segs - 1) / segs);
+        }
+        return 16;
+    }
     return rows;
 }
 static int stream_rows(const lbmx_lattice *L, int depth)
 {
+    static const int rows_env = getenv("LBMX_STREAM_ROWS") ? atoi(getenv("LBMX_STREAM_ROWS")) : 0;
+    if (rows_env > 0) return std::min(rows_env, L->Ny);
     return stream_rows_of(L->Nx, L->Ny, depth);
 }
 static long long stream_threshold(int precision, int depth)

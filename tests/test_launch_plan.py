@@ -31,9 +31,13 @@ def test_auto_rule(Nx, Ny, prec, depth):
         assert cols == 64 - 2 * (d - 1)
         strips = -(-Nx // cols)
         lo, hi = (32, 128) if d == 4 else (128, 512)
-        if d == 4 and rows == 16:
-            # 16-row segments only where 32-row ones give fewer than 768 workgroups
+        if d == 4 and rows < 32:
+            # 16-row segments only where 32-row ones give fewer than 768 workgroups,
+            # lengthened to fit one round of 1,024 workgroups when 16 rows give up to
+            # 1.5 times that many
             assert strips * -(-Ny // 32) < 768
+            w16 = strips * -(-Ny // 16)
+            assert rows == 16 or 1024 < w16 <= 1536
             base = 32
         else:
             assert rows == min(Ny, rows) and (rows & (rows - 1)) == 0 and lo <= rows <= hi
@@ -66,11 +70,21 @@ def test_tile_rows(N, rows):
 
 
 @pytest.mark.parametrize("N,prec,rows", [
-    (768, native.F64, 16), (1024, native.F64, 16), (1280, native.F64, 32),
-    (1024, native.F32_SHIFTED, 16), (1280, native.F32_SHIFTED, 32),
-    (1024, native.F32_WIDE, 16), (1280, native.F32_WIDE, 32)])
+    (768, native.F64, 16), (1024, native.F64, 19), (1280, native.F64, 32),
+    (1024, native.F32_SHIFTED, 19), (1280, native.F32_SHIFTED, 32),
+    (1024, native.F32_WIDE, 19), (1280, native.F32_WIDE, 32)])
 def test_small_stream_rows(N, prec, rows):
     assert native.launch_plan(N, N, prec) == (4, 58, rows)
+
+
+def test_one_round_rows():
+    # 16-row segments would give 1,152 workgroups (a second, partial round of the
+    # 1,024 the CUs hold); longer ones fit one round
+    for Nx, Ny in ((1024, 1024), (2048, 512), (1100, 1000)):
+        d, cols, rows = native.launch_plan(Nx, Ny, native.F64)
+        strips = -(-Nx // cols)
+        assert strips * -(-Ny // 16) > 1024
+        assert strips * -(-Ny // rows) <= 1024 < strips * -(-Ny // (rows - 1))
 
 
 def test_headline_shape():

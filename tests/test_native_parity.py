@@ -196,12 +196,15 @@ def test_launch_modes_fp32_identical(sc, precision):
 
 
 @pytest.mark.parametrize("Nx,Ny,plan", [
-    # 18 strips x 65 segments of 16 rows, the last of each partial
-    (1000, 1030, (4, 58, 16)),
+    # 16 strips x 57 segments of 16 rows, the last of each partial
+    (900, 900, (4, 58, 16)),
+    # 18 strips x 55 segments of 19 rows (16 rows would need a second round of
+    # workgroups), the last of each partial
+    (1000, 1030, (4, 58, 19)),
     # 32-row segments (920 workgroups)
     (1280, 1280, (4, 58, 32)),
     # a narrow lattice: one full 58-column strip and a partial 6-column one
-    (64, 90000, (4, 58, 32))], ids=["16row", "32row", "narrow"])
+    (64, 90000, (4, 58, 32))], ids=["16row", "19row", "32row", "narrow"])
 def test_fp32_four_step_stream_identical(Nx, Ny, plan):
     """FP32 storage and collision at four steps per launch (the automatic choice
     below 6 M cells) rounds exactly as one step per launch, with a force, for the
@@ -220,8 +223,9 @@ def test_fp32_four_step_stream_identical(Nx, Ny, plan):
 def test_launch_depth_rule():
     """LBMX_LAUNCH_AUTO streams rows once the lattice gives k_stream 200
     workgroups of 32 rows (400 with FP64 collision on FP32 storage; FP32 eight
-    steps from 6 M cells), and runs the tile kernels on smaller lattices; lattices with walls
-    take one step per launch."""
+    steps from 6 M cells), and runs the tile kernels on smaller lattices; small
+    lattices with walls take one step per launch unless streaming is forced, when
+    the bounded multi-step launch takes them."""
     from lbc_amd import native
     cases = [(2048, 2048, 0, 4), (256, 256, 0, 3), (1024, 1024, 0, 4), (2048, 2048, 1, 4),
              (1024, 1024, 1, 4), (256, 256, 1, 3), (2048, 2048, 2, 4), (768, 768, 0, 4),
@@ -244,6 +248,8 @@ def test_launch_depth_rule():
     lat.set_geometry(mask, np.full((32, 64), 1.25), None)
     lat.set_boundaries([("periodic_x", 0)])
     assert lat.launch_depth() == 1
+    lat.set_launch(native.LAUNCH_STREAM)
+    assert lat.launch_depth() > 1
     lat.close()
 
 
