@@ -521,7 +521,7 @@ def vtk_output_times(lat, reps=3):
             "identical": same}
 
 
-def cylinder_bench(steps=50000):
+def cylinder_bench(steps=50000, shedding=True):
     """Channel past a cylinder, 2048 x 512, Re = 100: FP64 and FP32 storage with the
     FP64 collision, `steps` steps each from the inflow profile with the drag/lift
     history recorded on the device every step (the cylinder driver's {t, L, D}
@@ -567,6 +567,8 @@ def cylinder_bench(steps=50000):
     o32["energy_rel_diff"] = float(
         f"{abs(o32['kinetic_energy'] - out['fp64']['kinetic_energy']) / out['fp64']['kinetic_energy']:.3e}")
     out["vs_fp64"] = round(o32["MLUPS"] / out["fp64"]["MLUPS"], 3)
+    if not shedding:
+        return out
     # shedding needs longer than the configuration's run: FP64 for 200,000 steps
     # (156 convective times) with a wake kick, statistics over the last 20
     long_steps = 200000
@@ -770,6 +772,10 @@ def main():
     ap.add_argument("--dropin", type=int, default=100, metavar="STEPS",
                     help="main_lbm calls of the drop-in benchmark (0: skip)")
     ap.add_argument("--cylinder-steps", type=int, default=50000)
+    ap.add_argument("--cylinder-only", action="store_true",
+                    help="only run the channel past a cylinder (FP64, FP32 storage, FP64 from "
+                         "rest; no 200,000-step shedding run) and print its JSON -- for "
+                         "profiling its kernels")
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",
                     help="skip the Taylor-Green roofline sweep 1024^2 .. 16384^2")
     ap.add_argument("--liner", type=int, default=20000, metavar="STEPS",
@@ -798,6 +804,9 @@ def main():
     N = args.size
     cells = N * N
     log(f"device: {native.device_name()}")
+    if args.cylinder_only:
+        print(json.dumps(cylinder_bench(args.cylinder_steps, shedding=False)), flush=True)
+        return
 
     prec, bpc, ptag, pname = {
         "f64": (native.F64, 144, "fp64", "FP64"),
