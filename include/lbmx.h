@@ -149,14 +149,21 @@ LBMX_API int lbmx_step(lbmx_lattice *lat, int nsteps);
  *                       strip with one wavefront per time level
  * A run of n steps that enters this form spends its first step on an in-place
  * collision; the remaining steps go in launches of the mode's depth, the last
- * few in shallower ones.  Other lattices ignore the mode. */
+ * few in shallower ones.
+ * Lattices with walls, bodies ((interpolated) bounce-back) or regularized
+ * velocity/pressure columns at x = 0 and x = Nx - 1, and no entry that stays stale,
+ * run three steps per launch under LBMX_LAUNCH_STREAM, and under LBMX_LAUNCH_AUTO
+ * when large enough (about 200 workgroups of 58 columns x 32 rows); every call of
+ * theirs ends with 1..3 steps of one step per launch, so between calls both
+ * buffers hold the reference's fields.  LBMX_LAUNCH_SINGLE and _TILES keep them
+ * at one step per launch.  Other lattices ignore the mode. */
 #define LBMX_LAUNCH_AUTO 0
 #define LBMX_LAUNCH_SINGLE 1
 #define LBMX_LAUNCH_TILES 2
 #define LBMX_LAUNCH_STREAM 3
 LBMX_API int lbmx_set_launch(lbmx_lattice *lat, int mode);
 /* Steps per launch of a long run of this lattice under its mode (1 when the
- * lattice does not qualify). */
+ * lattice takes one step per launch). */
 LBMX_API int lbmx_launch_depth(const lbmx_lattice *lat);
 /* The same rule for a qualifying lattice of this size and precision, on the host
  * (no device needed): *steps_per_launch, and the output columns (*columns) and
