@@ -31,6 +31,17 @@ int env_int(const char *name, int dflt)
 
 }   // namespace
 
+// doubles of one half of chan_fpart: per step, pair and link block two values
+static size_t chan_fpart_half()
+{
+    return (size_t)kChanDepth * 2 * kChanFusedBlocks * 2;
+}
+
+static double *chan_ldf_at(const lbmx_lattice *L, int half)
+{
+    return L->chan_ldf ? L->chan_ldf + (size_t)half * kChanDepth * L->nlinks : nullptr;
+}
+
 void lbmx_chan_free(lbmx_lattice *L)
 {
     hipFree(L->chan_node);
@@ -51,6 +62,10 @@ void lbmx_chan_free(lbmx_lattice *L)
     L->chan_aux = nullptr;
     L->chan_done = nullptr;
     L->chan_ok = -1;
+    hipFree(L->chan_fpart);
+    L->chan_fpart = nullptr;
+    L->chan_ja = L->chan_jb = 0;
+    L->chan_ldf_half = 0;
 }
 
 // The geometry and boundary table allow the bounded launch: every program entry
@@ -158,6 +173,7 @@ static int chan_prepare(lbmx_lattice *L)
     if (hipMalloc((void **)&L->chan_node, nn * sizeof(uint2)) != hipSuccess ||
         hipMalloc((void **)&L->chan_aux, sizeof(ChanAux)) != hipSuccess ||
         hipMalloc((void **)&L->chan_done, sizeof(unsigned)) != hipSuccess ||
+        hipMalloc((void **)&L->chan_fpart, chan_fpart_half() * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&L->chan_part, (size_t)kChanDepth * (nblk > 0 ? nblk : 1) *
                                               kPartial * sizeof(double)) != hipSuccess) {
         lbmx_chan_free(L);
@@ -165,7 +181,7 @@ static int chan_prepare(lbmx_lattice *L)
     }
     if (!lq.empty() &&
         (hipMalloc((void **)&L->chan_lq, lq.size() * sizeof(double)) != hipSuccess ||
-         hipMalloc((void **)&L->chan_ldf, (size_t)kChanDepth * lq.size() * sizeof(double)) !=
+         hipMalloc((void **)&L->chan_ldf, 2 * (size_t)kChanDepth * lq.size() * sizeof(double)) !=
              hipSuccess)) {
         lbmx_chan_free(L);
         return lbmx_fail(LBMX_ERR_NOMEM, "bounded launch link tables");
@@ -203,7 +219,6 @@ static int chan_write_aux(lbmx_lattice *L)
 {
     ChanAux x{};
     x.lq = L->chan_lq;
-    x.ldf = L->chan_ldf;
     x.nlinks = L->nlinks;
     if (L->chan_lo >= 0) {
         const BcDesc &d = L->prog_desc[L->chan_lo];
@@ -255,30 +270,54 @@ static ChanArgs chan_args(const lbmx_lattice *L, const StepFlags &s)
     return a;
 }
 
-// Forces of the launch's steps from ldf; record rows from frec_row on.  (Run on
-// a second stream to overlap the next launch, the cross-stream waits cost more
-// than the 10 us kernel: cylinder 2048 x 512 FP64 30.7 vs 22.9 us per step.)
-static int chan_forces(lbmx_lattice *L, int nlev)
+// The forces of a launch's steps (terms in ldf, record rows from row0 on).  With
+// at most kChanFusedBlocks link blocks the next two launches of the call sum them
+// in their force workgroups (stages A and B), and k_chan_force sums those of the
+// call's last two launches when it leaves the bounded launches (lbmx_chan_flush);
+// with more, k_chan_force runs after every launch.
+static ChanForceArgs chan_force_args(const lbmx_lattice *L, const double *ldf, int nlev,
+                                     long long row0)
 {
-    if (L->nlinks == 0 && !L->frec) return LBMX_OK;
     ChanForceArgs f{};
     f.links = L->links;
     f.nlinks = L->nlinks;
-    f.ldf = L->chan_ldf;
+    f.ldf = ldf;
     f.nlev = nlev;
     f.partials = L->chan_part;
     f.nblk = L->nlinks ? L->npartials : 0;
     f.done = L->chan_done;
     f.force_out = L->force_dev;
     f.record = L->frec;
-    f.row0 = L->frec_row;
+    f.row0 = row0;
     f.capacity = L->frec_cap;
     f.dt = L->frec_dt;
     f.cf = L->frec_cf;
+    return f;
+}
+
+static int chan_force_kernel(lbmx_lattice *L, const ChanForceArgs &f)
+{
     hipLaunchKernelGGL(k_chan_force, dim3(f.nblk > 0 ? f.nblk : 1), dim3(kLinkThreads), 0,
                        L->stream, f);
     HIPCHK(hipGetLastError());
-    if (L->frec) L->frec_row += nlev;
+    return LBMX_OK;
+}
+
+int lbmx_chan_flush(lbmx_lattice *L)
+{
+    // the older launch first: force_out ends with the newest step
+    if (L->chan_jb) {
+        L->chan_jb = 0;
+        const ChanForceArgs f =
+            chan_force_args(L, chan_ldf_at(L, L->chan_jb_half), kChanDepth, L->chan_jb_row);
+        if (int rc = chan_force_kernel(L, f)) return rc;
+    }
+    if (L->chan_ja) {
+        L->chan_ja = 0;
+        const ChanForceArgs f =
+            chan_force_args(L, chan_ldf_at(L, L->chan_ja_half), kChanDepth, L->chan_ja_row);
+        if (int rc = chan_force_kernel(L, f)) return rc;
+    }
     return LBMX_OK;
 }
 
@@ -340,7 +379,9 @@ static int chan_work(lbmx_lattice *L, int NL, int TX, int per_cu, ChanArgs &a)
             }
         }
     };
-    const long long slots = (long long)per_cu * (L->ncu > 0 ? L->ncu : 256);
+    // the launch's force workgroups take slots too
+    const long long slots =
+        (long long)per_cu * (L->ncu > 0 ? L->ncu : 256) - 2 * kChanDepth * kChanSplit;
     std::vector<int4> list;
     if (rows_env > 0) {
         build(std::min(rows_env, Ny), list);
@@ -372,7 +413,7 @@ static int chan_work(lbmx_lattice *L, int NL, int TX, int per_cu, ChanArgs &a)
 }
 
 template <typename T, bool W, int P, int R>
-static int launch_chan(lbmx_lattice *L, T *S, T *Dst, ChanArgs &a)
+static int launch_chan(lbmx_lattice *L, T *S, T *Dst, ChanArgs &a, const ChanFusedArgs &fj)
 {
     constexpr int NL = kChanDepth + 1;
     constexpr int TX = ChanShape<NL, kChanLW, R>::TX;
@@ -388,9 +429,10 @@ static int launch_chan(lbmx_lattice *L, T *S, T *Dst, ChanArgs &a)
     }
     if (int rc = chan_work(L, NL, TX, per_cu, a)) return rc;
     a.work = L->chan_work;
-    const unsigned blocks = ((unsigned)a.nwork + 7) / 8 * 8;
+    const unsigned nf = (fj.a_on || fj.b_on) ? 2u * kChanDepth * kChanSplit : 0u;
+    const unsigned blocks = ((unsigned)a.nwork + 7) / 8 * 8 + nf;
     hipLaunchKernelGGL((k_chan<T, W, NL, kChanLW, P, R>), dim3(blocks), dim3(kChanLW * NL), 0,
-                       L->stream, S, Dst, a);
+                       L->stream, S, Dst, a, fj);
     return LBMX_OK;
 }
 
@@ -401,15 +443,55 @@ static int chan_run_t(lbmx_lattice *L)
     ChanArgs a = chan_args(L, s);
     T *S = (T *)L->pop[L->cur];
     T *Dst = (T *)L->pop[L->cur ^ 1];
+    // this launch's link terms go to one half of ldf while its stage A reads the
+    // previous launch's from the other
+    a.ldf = chan_ldf_at(L, L->chan_ldf_half);
+    const bool fused = L->npartials <= kChanFusedBlocks;
+    ChanFusedArgs fj{};
+    if (L->chan_jb) {
+        fj.b_on = 1;
+        fj.b_nblk = L->nlinks ? L->npartials : 0;
+        fj.part_in = L->chan_fpart + L->chan_jb_half * chan_fpart_half();
+        fj.force_out = L->force_dev;
+        fj.record = L->frec;
+        fj.row0 = L->chan_jb_row;
+        fj.capacity = L->frec_cap;
+        fj.dt = L->frec_dt;
+        fj.cf = L->frec_cf;
+    }
+    if (L->chan_ja && L->nlinks) {
+        fj.a_on = 1;
+        fj.links = L->links;
+        fj.nlinks = L->nlinks;
+        fj.nblk = L->npartials;
+        fj.ldf = chan_ldf_at(L, L->chan_ja_half);
+        fj.part_out = L->chan_fpart + L->chan_ja_half * chan_fpart_half();
+    }
     int rc;
     if constexpr (sizeof(T) == 8)
-        rc = launch_chan<T, false, 3, 3>(L, S, Dst, a);
+        rc = launch_chan<T, false, 3, 3>(L, S, Dst, a, fj);
     else if (L->cfg.precision == LBMX_F32_WIDE)
-        rc = launch_chan<T, true, 4, 4>(L, S, Dst, a);
+        rc = launch_chan<T, true, 4, 4>(L, S, Dst, a, fj);
     else
-        rc = launch_chan<T, false, 4, 4>(L, S, Dst, a);
+        rc = launch_chan<T, false, 4, 4>(L, S, Dst, a, fj);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
+    L->chan_jb = L->chan_ja;
+    L->chan_jb_half = L->chan_ja_half;
+    L->chan_jb_row = L->chan_ja_row;
+    L->chan_ja = 0;
+    if (L->nlinks || L->frec) {
+        const long long row = L->frec_row;
+        if (L->frec) L->frec_row += kChanDepth;
+        if (fused) {
+            L->chan_ja = 1;
+            L->chan_ja_half = L->chan_ldf_half;
+            L->chan_ja_row = row;
+        } else if (int rf = chan_force_kernel(L, chan_force_args(L, a.ldf, kChanDepth, row))) {
+            return rf;
+        }
+    }
+    L->chan_ldf_half ^= 1;
     L->cur ^= 1;
     L->other_is_ref = 0;
     s.Fx_lag = L->cfg.Fx;
@@ -419,11 +501,12 @@ static int chan_run_t(lbmx_lattice *L)
     L->Fx_lag = L->cfg.Fx;
     L->Fy_lag = L->cfg.Fy;
     L->steps += kChanDepth;
-    return chan_forces(L, kChanDepth);
+    return LBMX_OK;
 }
 
 // One launch of lbmx_chan_depth() steps from the one-step path's state.  The
-// caller runs at least one step of the one-step path afterwards.
+// caller calls lbmx_chan_flush after its last launch, then runs at least one step
+// of the one-step path.
 int lbmx_chan_run(lbmx_lattice *L)
 {
     if (int rc = chan_prepare(L)) return rc;

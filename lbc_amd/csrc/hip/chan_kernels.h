@@ -52,7 +52,6 @@ namespace lbmx {
 // Tables of the rare paths (device memory, one per lattice).
 struct ChanAux {
     const double *lq;         // per link: q, or -1 where the value is plain bounce-back
-    double *ldf;              // [D][nlinks]: f*(x,k) + f_new(x, opp k) of each step
     int nlinks;
     int lo_type, hi_type;     // regularized column at x = 0 / x = Nx - 1 (0: none)
     const double *lo_v1, *lo_v2, *hi_v1, *hi_v2;
@@ -77,6 +76,7 @@ struct ChanArgs {
     // when x + c_k is solid, k = 1..8); y = index of the node's first link
     const uint2 *node;
     const ChanAux *aux;
+    double *ldf;              // [D][nlinks]: f*(x,k) + f_new(x, opp k) of each step
     // per workgroup: first output column, rows [y0, y1), and 0 in .w when its region
     // (strip and rows with the levels' rings) holds only fluid nodes of the uniform
     // omega, away from closed edges and columns
@@ -328,7 +328,7 @@ __device__ __forceinline__ void chan_level(T *buf, double *mring, T *__restrict_
                         };
                         const ChanAux &x_ = *a.aux;
                         chan_links(p, lm, (int)npre[j].y, x_.lq, a.rho0, star,
-                                   own ? x_.ldf + (size_t)(L - 2) * (size_t)x_.nlinks : nullptr);
+                                   own ? a.ldf + (size_t)(L - 2) * (size_t)x_.nlinks : nullptr);
                     }
                 }
                 const bool fluid = (code & 0xffu) != 0u;
@@ -515,22 +515,214 @@ __device__ __forceinline__ void chan_plain_first(T *buf, const T *__restrict__ S
     }
 }
 
+// ---- forces of the steps a launch covered ---------------------------------------
+// Level l (2..NL) of a launch from f_k stores the momentum-exchange terms of step
+// k + l - 2 per link in ldf; k_chan_force forms, per step, the link blocks'
+// partial sums (block_sum over 256 links in list order, as links_block), and the
+// last block to finish sums the partials as the one-step path's first k_tail
+// launch does (bc_force_total with kTailThreads threads), writing the force
+// outputs and record rows.  Same terms, same association: the same forces bit for
+// bit.
+struct ChanForceArgs {
+    const Link *links;
+    int nlinks;
+    const double *ldf;     // [nlev][nlinks]
+    int nlev;
+    double *partials;      // [nlev][nblk][kPartial]
+    int nblk;              // link blocks (0: no links; one block writes zero rows)
+    unsigned *done;        // blocks finished (the last one resets it)
+    double *force_out;
+    double *record;        // rows {t, L, D, L_body, D_body}
+    long long row0;
+    int capacity;
+    double dt, cf;
+};
+
+constexpr int kChanMaxLev = 4;   // steps one force launch sums at most
+
+// Lattice direction components from bit masks of k (the d2q9 tables' values:
+// c_x = +1 for k = 1, 5, 8 and -1 for 3, 6, 7; c_y = +1 for 2, 5, 6 and -1 for
+// 4, 7, 8), so that a lane's direction needs no table load and no branch.
+__device__ __forceinline__ int chan_cx(int k)
+{
+    return (int)((0x122u >> k) & 1u) - (int)((0x0c8u >> k) & 1u);
+}
+__device__ __forceinline__ int chan_cy(int k)
+{
+    return (int)((0x064u >> k) & 1u) - (int)((0x190u >> k) & 1u);
+}
+
+// v of lane i + 32 (lanes 0 .. 31) or of lane i + 16 (lanes 0 .. 15 and 32 .. 47):
+// the second result of v_permlane32_swap / v_permlane16_swap with v as both
+// operands (the upper half, or the odd rows, of the first operand swapped with the
+// lower half, or the even rows, of the second).
+__device__ __forceinline__ double chan_from_upper32(double v)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32),
+                                                     false, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+__device__ __forceinline__ double chan_from_upper16(double v)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32),
+                                                     false, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+
+// Lane 0's value of the 64-lane xor butterfly of block_sum (v += v of lane ^ o for
+// o = 32, 16, ..., 1).  At stage o only lanes below o feed lane 0's later stages,
+// and each of them adds the value of lane + o, which it takes by a lane swap (32,
+// 16) or a row shift (8 .. 1) with no LDS traffic; every addition is the one the
+// butterfly makes, so lane 0 ends with the butterfly's sum bit for bit.  Other
+// lanes end with partial values.
+__device__ __forceinline__ double chan_wave_sum(double v)
+{
+    v += chan_from_upper32(v);
+    v += chan_from_upper16(v);
+    v += dpp_shift<0x108>(v);   // row_shl:8: lane + 8
+    v += dpp_shift<0x104>(v);   // row_shl:4
+    v += dpp_shift<0x102>(v);   // row_shl:2
+    v += dpp_shift<0x101>(v);   // row_shl:1
+    return v;
+}
+
+// Forces inside the launches, for lattices of at most kChanFusedBlocks link
+// blocks: a launch's link terms are summed by the next two launches of the call,
+// which need no workgroup to wait for another.  Stage A (in launch k + 1) forms
+// every link block's block_sum value per step and component pair into part; stage
+// B (in launch k + 2) forms each step's total from them as k_chan_force's last
+// block does and writes the force outputs and record rows.
+constexpr int kChanFusedBlocks = 128;
+constexpr int kChanSplit = 2;   // force workgroups per step and component pair
+
+struct ChanFusedArgs {
+    // stage A: terms of the launch before (ldf) -> part_out
+    int a_on;
+    const Link *links;
+    int nlinks, nblk;      // nblk <= kChanFusedBlocks
+    const double *ldf;     // [nlev][nlinks]
+    double *part_out;      // [nlev][2][kChanFusedBlocks][2]
+    // stage B: part_in (stage A of the launch before) -> totals
+    int b_on;
+    int b_nblk;
+    const double *part_in;
+    double *force_out;
+    double *record;        // rows {t, L, D, L_body, D_body}
+    long long row0;
+    int capacity;
+    double dt, cf;
+};
+
+// Stage A in group g (NLEV * 2 * kChanSplit of them): step g / (2 kChanSplit),
+// the total's (x, y) components (pair 0) or the body's (pair 1); wave
+// j = (g % kChanSplit) * 4 + w of the pair's takes the link blocks b = j,
+// j + 4 kChanSplit, ...: the four 64-link butterflies, their sums added in order
+// to 0 (links_block's block_sum).
+template <int NLEV>
+__device__ __forceinline__ void chan_force_partials(const ChanFusedArgs &a, int g)
+{
+    constexpr int NW = kLinkThreads / 64;
+    const int lev = g / (2 * kChanSplit), pair = (g / kChanSplit) % 2;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int stride = kChanSplit * NW;
+    const double *ldf = a.ldf + (size_t)lev * a.nlinks;
+    double *out = a.part_out + (size_t)(lev * 2 + pair) * kChanFusedBlocks * 2;
+    for (int b = (g % kChanSplit) * NW + wave; b < a.nblk; b += stride) {
+        int k[NW], body[NW];
+        double d[NW];
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            // lanes past the last link read the last one and add zeros
+            const int l = min(b * kLinkThreads + i * 64 + lane, a.nlinks - 1);
+            k[i] = a.links[l].k;
+            body[i] = a.links[l].body;
+            d[i] = ldf[l];
+        }
+        double sum[2] = {0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            const bool have = b * kLinkThreads + i * 64 + lane < a.nlinks;
+            const bool take = have && (pair == 0 || body[i] != 0);
+            const double f0 = take ? d[i] * (double)chan_cx(k[i]) : 0.0;
+            const double f1 = take ? d[i] * (double)chan_cy(k[i]) : 0.0;
+            sum[0] += chan_wave_sum(f0);
+            sum[1] += chan_wave_sum(f1);
+        }
+        if (lane == 0) {
+            out[2 * b] = sum[0];
+            out[2 * b + 1] = sum[1];
+        }
+    }
+}
+
+// Stage B of step lev, pair (wave 0): lane t adds the block values t, t + 64 (each
+// to 0, as bc_force_total<128>'s loops do), then shuffles down by 32 .. 1:
+// k_chan_force's association, the one-step path's sums.
+template <int NLEV>
+__device__ __forceinline__ void chan_force_totals(const ChanFusedArgs &a, int lev, int pair)
+{
+    static_assert(kChanFusedBlocks == 2 * 64, "lane t sums blocks t and t + 64");
+    const int lane = threadIdx.x;
+    const double *in = a.part_in + (size_t)(lev * 2 + pair) * kChanFusedBlocks * 2;
+    double v[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        double sa = 0.0, sb = 0.0;
+        if (lane < a.b_nblk) sa += in[2 * lane + c];
+        if (lane + 64 < a.b_nblk) sb += in[2 * (lane + 64) + c];
+        v[c] = sa + sb;
+    }
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1)
+#pragma unroll
+        for (int c = 0; c < 2; c++) v[c] += __shfl_down(v[c], w, 64);
+    if (lane != 0) return;
+    if (a.b_nblk > 0 && lev == NLEV - 1) {
+        a.force_out[2 * pair] = v[0];
+        a.force_out[2 * pair + 1] = v[1];
+    }
+    const long long row = a.row0 + lev;
+    if (a.record && row < a.capacity) {
+        double *r = a.record + (size_t)kRecordCols * row;
+        if (pair == 0) r[0] = (double)row * a.dt;
+        r[1 + 2 * pair] = v[1] * a.cf;
+        r[2 + 2 * pair] = v[0] * a.cf;
+    }
+}
+
 // NL - 1 steps per launch; S holds f_k (only its solid nodes are written), Dst
-// receives f_{k+NL-1} at the fluid nodes.
+// receives f_{k+NL-1} at the fluid nodes.  With fj.a_on or fj.b_on the first
+// 2 (NL - 1) kChanSplit blocks are force workgroups: group g runs stage B of its
+// step and pair when g % kChanSplit == 0, then its share of stage A; the work
+// items follow.
 template <typename T, bool W, int NL, int LW, int P, int R>
 __global__ void __launch_bounds__(LW * NL) __attribute__((amdgpu_waves_per_eu(4)))
-k_chan(T *__restrict__ S, T *__restrict__ Dst, ChanArgs a)
+k_chan(T *__restrict__ S, T *__restrict__ Dst, ChanArgs a, ChanFusedArgs fj)
 {
     using Sh = ChanShape<NL, LW, R>;
     static_assert(LW == 64 && NL >= 2 && Sh::TX > 0 && (R == 3 || R == 4), "strip shape");
+    static_assert(LW * NL == kLinkThreads, "a force workgroup is one link block wide");
     __shared__ T buf[Sh::lds];
     __shared__ double mring[Sh::mring];
+    const int nf = (fj.a_on || fj.b_on) ? 2 * (NL - 1) * kChanSplit : 0;
+    if ((int)blockIdx.x < nf) {
+        const int g = blockIdx.x;
+        if (fj.b_on && g % kChanSplit == 0 && threadIdx.x < 64)
+            chan_force_totals<NL - 1>(fj, g / (2 * kChanSplit), (g / kChanSplit) % 2);
+        if (fj.a_on) chan_force_partials<NL - 1>(fj, g);
+        return;
+    }
     const int level = __builtin_amdgcn_readfirstlane(threadIdx.x / LW) + 1;
     const int lane = threadIdx.x % LW;
-    // XCD-aware order: consecutive workgroups go to the 8 XCDs in turn, so XCD i
-    // takes the i-th eighth of the work list (strips of one row band, then the next)
+    // XCD-aware order: consecutive workgroups go to the 8 XCDs in turn, so each XCD
+    // takes one eighth of the work list (strips of one row band, then the next)
+    const int bw = (int)blockIdx.x - nf;
     const int per = (a.nwork + 7) / 8;
-    const int item = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int item = (bw % 8) * per + bw / 8;
     if (item >= a.nwork) return;   // whole workgroup, before any barrier
     const int4 wk = a.work[item];
     const int x0 = wk.x, y0 = wk.y, y1 = wk.z;
@@ -604,31 +796,6 @@ k_chan(T *__restrict__ S, T *__restrict__ Dst, ChanArgs a)
                                         std::make_integer_sequence<int, NL - 1>{});
     }
 }
-
-// ---- forces of the steps a launch covered ---------------------------------------
-// Level l (2..NL) of a launch from f_k stores the momentum-exchange terms of step
-// k + l - 2 per link in ldf; k_chan_force forms, per step, the link blocks'
-// partial sums (block_sum over 256 links in list order, as links_block), and the
-// last block to finish sums the partials as the one-step path's first k_tail
-// launch does (bc_force_total with kTailThreads threads), writing the force
-// outputs and record rows.  Same terms, same association: the same forces bit for
-// bit.
-struct ChanForceArgs {
-    const Link *links;
-    int nlinks;
-    const double *ldf;     // [nlev][nlinks]
-    int nlev;
-    double *partials;      // [nlev][nblk][kPartial]
-    int nblk;              // link blocks (0: no links; one block writes zero rows)
-    unsigned *done;        // blocks finished (the last one resets it)
-    double *force_out;
-    double *record;        // rows {t, L, D, L_body, D_body}
-    long long row0;
-    int capacity;
-    double dt, cf;
-};
-
-constexpr int kChanMaxLev = 4;   // steps one force launch sums at most
 
 // Per step, thread t of block b takes link b * 256 + t; the step's block partial
 // is links_block's block_sum (a butterfly within each wavefront, then the

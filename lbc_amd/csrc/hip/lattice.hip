@@ -1671,8 +1671,11 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
     if (!L->pull && (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && lbmx_chan_fits(L))) &&
         nsteps > lbmx_chan_depth() && lbmx_chan_eligible(L)) {
         const int cd = lbmx_chan_depth();
-        for (const int n = (nsteps - 1) / cd; i < n * cd; i += cd)
-            if (int rc = lbmx_chan_run(L)) return rc;
+        int rc = LBMX_OK;
+        for (const int n = (nsteps - 1) / cd; rc == LBMX_OK && i < n * cd; i += cd)
+            rc = lbmx_chan_run(L);
+        const int rf = lbmx_chan_flush(L);   // the last launch's forces
+        if (rc || rf) return rc ? rc : rf;
     }
     // tile depth: 3 steps per launch (the tiles serve the lattices too small for
     // row streaming, where three beat two in FP64 as in FP32: 256^2 3.69 vs 3.97

@@ -105,6 +105,14 @@ struct lbmx_lattice {
     int4 *chan_work = nullptr;       // work items of a launch: x0, y0, y1, bounded path
     long long chan_work_key = -1;    // the launch shape chan_work was built for
     int chan_nwork = 0, chan_nspecial = 0;
+    // forces summed inside the launches: a launch's link terms (one half of
+    // chan_ldf) go through stage A in the next launch (block values into the same
+    // half of chan_fpart) and stage B in the one after (totals, record rows)
+    int chan_ldf_half = 0;           // half of chan_ldf the next launch writes
+    double *chan_fpart = nullptr;    // two halves of per-block values
+    int chan_ja = 0, chan_jb = 0;    // a launch awaits stage A (and B) / stage B
+    int chan_ja_half = 0, chan_jb_half = 0;
+    long long chan_ja_row = 0, chan_jb_row = 0;   // their first record rows
     std::vector<unsigned> chan_plain_rows;   // per row: prefix counts of non-plain nodes
     int nclasses = 0;                // fluid classes (geom 1)
     double hlut1 = 1.0;              // omega of class 1
@@ -157,4 +165,5 @@ void lbmx_chan_free(lbmx_lattice *L);
 int lbmx_chan_eligible(lbmx_lattice *L);
 int lbmx_chan_fits(const lbmx_lattice *L);
 int lbmx_chan_run(lbmx_lattice *L);
+int lbmx_chan_flush(lbmx_lattice *L);
 int lbmx_chan_depth();
