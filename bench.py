@@ -89,6 +89,8 @@ FP64_KERNELS = {
     2: "k_pull2<double, false, 8> (two steps per launch, 64 x 8 tiles)",
     3: "k_pull3<double, false, 8> (three steps per launch, 64 x 8 tiles)",
     4: "k_stream<double, false, 4, 64, 3, 3> (four steps per launch, row streaming)",
+    5: "k_stream<double, false, 5, 64, 3, 3, true> (five steps per launch, row streaming, "
+       "force-free)",
 }
 
 

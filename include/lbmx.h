@@ -144,9 +144,11 @@ LBMX_API int lbmx_step(lbmx_lattice *lat, int nsteps);
  *                       environment sets LBMX_NO_TEMPORAL)
  *   LBMX_LAUNCH_TILES   three steps per launch (two with LBMX_TEMPORAL_DEPTH=2) on
  *                       64 x TY tiles with their rings in LDS
- *   LBMX_LAUNCH_STREAM  4 (FP64, FP32 storage with FP64 collision) or 8 (FP32)
- *                       steps per launch, each workgroup streaming down a column
- *                       strip with one wavefront per time level
+ *   LBMX_LAUNCH_STREAM  5 (FP64), 4 (FP32 storage with FP64 collision) or 8
+ *                       (FP32) steps per launch, each workgroup streaming down a
+ *                       column strip with one wavefront per time level (under
+ *                       LBMX_LAUNCH_AUTO, FP64 takes 4 on mid-size lattices and FP32
+ *                       4 below 6 M cells)
  * A run of n steps that enters this form spends its first step on an in-place
  * collision; the remaining steps go in launches of the mode's depth, the last
  * few in shallower ones.

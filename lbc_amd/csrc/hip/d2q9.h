@@ -138,4 +138,54 @@ __device__ __forceinline__ void collide_ref(double f[Q], const Moments &m, doubl
     collide_all(f, m, usq, om, force, Fx, Fy, uF, std::make_integer_sequence<int, Q>{});
 }
 
+// ---- lattices without a body force ---------------------------------------------
+// With Fx = Fy = 0 in every step the reference adds 0.5*F = +0 to the momentum
+// sums (compute_macroscopic_fields.c:38-39) and a Guo term of +-0 to each update
+// (collision.c:34-37).  Both can change only the sign of an exact zero, and no
+// later operation of the step sees that sign: a zero velocity component enters the
+// equilibrium only through c.u (then 1 + 3c.u, a sum with 1) and squares, so
+// moments_nf / collide_nf give moments_ref / collide_ref's populations bit for bit.
+__device__ __forceinline__ Moments moments_nf(const double f[Q])
+{
+    double s = f[0];
+#pragma unroll
+    for (int k = 1; k < Q; k++) s += f[k];
+    const double sx = ((((f[1] - f[3]) + f[5]) - f[6]) - f[7]) + f[8];
+    const double sy = ((((f[2] - f[4]) + f[5]) + f[6]) - f[7]) - f[8];
+    double r = s;
+    if (r < kRhoFloor) r = kRhoFloor;
+    return Moments{r, sx / r, sy / r};
+}
+
+// Opposite directions k and opp(k) see c.u = +-c (cdot gives the opposite
+// direction's sum as the exact negation), and under round-to-nearest
+// 3(-c) = -(3c) and (4.5(-c))(-c) = (4.5c)c exactly, so the reference's
+// w rho (1 + 3cu + 4.5cu cu - 1.5u.u) of the pair is formed from one 3c and one
+// 4.5c*c: the same roundings on the same operands, with fewer operations.
+template <int K>
+__device__ __forceinline__ void collide_nf_pair(double f[Q], double c, double wr, double t15,
+                                                double nom)
+{
+    constexpr int K2 = D2Q9::opp[K];
+    const double a = 3.0 * c;
+    const double b = 4.5 * c * c;
+    const double ea = wr * (((1.0 + a) + b) - t15);
+    const double eb = wr * (((1.0 - a) + b) - t15);
+    f[K] = f[K] + nom * (f[K] - ea);
+    f[K2] = f[K2] + nom * (f[K2] - eb);
+}
+
+__device__ __forceinline__ void collide_nf(double f[Q], const Moments &m, double om)
+{
+    const double usq = m.ux * m.ux + m.uy * m.uy;
+    const double t15 = 1.5 * usq;
+    const double nom = -om;
+    const double w0r = D2Q9::w[0] * m.rho, w1r = D2Q9::w[1] * m.rho, w5r = D2Q9::w[5] * m.rho;
+    f[0] = f[0] + nom * (f[0] - w0r * (1.0 - t15));
+    collide_nf_pair<1>(f, m.ux, w1r, t15, nom);                        // (1, 3): c.u = ux
+    collide_nf_pair<2>(f, m.uy, w1r, t15, nom);                        // (2, 4): uy
+    collide_nf_pair<5>(f, cdot<1, 1>(m.ux, m.uy), w5r, t15, nom);      // (5, 7): ux + uy
+    collide_nf_pair<6>(f, cdot<-1, 1>(m.ux, m.uy), w5r, t15, nom);     // (6, 8): uy - ux
+}
+
 }  // namespace lbmx

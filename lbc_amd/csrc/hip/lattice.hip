@@ -1284,38 +1284,58 @@ static int pull_step(lbmx_lattice *L)
     return LBMX_OK;
 }
 
+// A launch with no body force in its steps or in its first step's lagged moments
+// runs the kernels' force-free instantiation (the same populations bit for bit).
+template <typename A>
+static bool no_force(const A &a)
+{
+    return !a.force && a.Fx == 0.0 && a.Fy == 0.0 && a.Fx_lag == 0.0 && a.Fy_lag == 0.0;
+}
+
 // Two gather-form steps in one launch (k_pull2); f*_{s+1} stays in LDS and only
 // f*_{s+2} is stored.
 template <typename T, bool W, int TY>
 static void launch_pull2(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
 {
     const unsigned blocks = (unsigned)a.tiles_x * (unsigned)((L->Ny + TY - 1) / TY);
-    hipLaunchKernelGGL((k_pull2<T, W, TY>), dim3(blocks), dim3(kPull2Threads), 0, L->stream, S,
-                       D, a);
+    if (no_force(a))
+        hipLaunchKernelGGL((k_pull2<T, W, TY, true>), dim3(blocks), dim3(kPull2Threads), 0,
+                           L->stream, S, D, a);
+    else
+        hipLaunchKernelGGL((k_pull2<T, W, TY>), dim3(blocks), dim3(kPull2Threads), 0, L->stream,
+                           S, D, a);
 }
 
 template <typename T, bool W, int TY>
 static void launch_pull3(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
 {
     const unsigned blocks = (unsigned)a.tiles_x * (unsigned)((L->Ny + TY - 1) / TY);
-    hipLaunchKernelGGL((k_pull3<T, W, TY>), dim3(blocks), dim3(kPull2Threads), 0, L->stream, S,
-                       D, a);
+    if (no_force(a))
+        hipLaunchKernelGGL((k_pull3<T, W, TY, true>), dim3(blocks), dim3(kPull2Threads), 0,
+                           L->stream, S, D, a);
+    else
+        hipLaunchKernelGGL((k_pull3<T, W, TY>), dim3(blocks), dim3(kPull2Threads), 0, L->stream,
+                           S, D, a);
 }
 
 // ---- row streaming (k_stream): D steps per launch ---------------------------------
 // Depth, prefetch, ring and rows per workgroup were measured on the periodic
-// Taylor-Green lattice at 1024^2 .. 16384^2: FP64 4 steps (three-row rings, LDS
-// 40.2 KB, four workgroups per CU), FP32 storage 8 steps with FP32 collision (two
-// workgroups of 8 wavefronts) and 4 with FP64 collision.
+// Taylor-Green lattice at 1024^2 .. 16384^2: FP64 5 steps on large lattices (slim
+// three-row rings, LDS 35.1 KB, four workgroups per CU) and 4 below, FP32 storage
+// 8 steps with FP32 collision (two workgroups of 8 wavefronts) and 4 with FP64
+// collision.
 constexpr int kStreamLW = 64;
 
 // Rows per workgroup: about 4096 workgroups over the lattice, as a power of two
 // between 32 and 128 at depth 4, and between 128 (the depth-8 pipeline takes 21
-// iterations to fill) and 512 at depth 8.
+// iterations to fill) and 512 at depth 8.  Depth 5 (FP64): 256 rows where that
+// still gives 4096 workgroups, else 128 (GLUPS 128 / 256 / 512 rows: 16384^2
+// 125.8 / 127.7-132.1 / 128.5, 8192^2 126.6 / 129.3, 4096^2 124.6 / 111.5).
 static int stream_rows_base(int Nx, int Ny, int depth)
 {
     const int tx = kStreamLW - 2 * (depth - 1);
     const double strips = (double)((Nx + tx - 1) / tx);
+    if (depth == 5) return std::min(strips * ((Ny + 255) / 256) >= 4096 ? 256 : 128, Ny);
     const double want = strips * Ny / 4096.0;
     const int lo = depth <= 4 ? 32 : 128, hi = depth <= 4 ? 128 : 512;
     int rows = lo;
@@ -1375,8 +1395,16 @@ static long long stream_threshold(int precision, int depth)
 // faster one while the two FP32 population buffers mostly fit the 256 MB Infinity
 // Cache, GLUPS four vs eight steps: 1408^2 163 vs 127, 1536^2 175 vs 151, 2048^2
 // 192 vs 173; 3072^2 168 vs 205, 4096^2 153 vs 206, 8192^2 154 vs 218.
+static long long stream_wgs_of(int Nx, int Ny, int depth);
+// FP64 streams five steps per launch where the five-level grid has 512 workgroups
+// of 128 rows (from about 2048 x 1800), and always when streaming is forced; four
+// below.  GLUPS four vs five steps: 2048^2 100.4 vs 108.2, 4096^2 ~104 vs 124.6,
+// 8192^2 98-112 vs 129.3, 16384^2 101-103 vs 127.7-132.1 (six steps: 119.9, three
+// workgroups per CU).
 static int stream_depth_of(int Nx, int Ny, int precision, int mode)
 {
+    if (precision == LBMX_F64)
+        return (mode == LBMX_LAUNCH_STREAM || stream_wgs_of(Nx, Ny, 5) >= 512) ? 5 : 4;
     if (precision != LBMX_F32_SHIFTED) return 4;
     if (mode == LBMX_LAUNCH_STREAM || (long long)Nx * Ny >= 6000000) return 8;
     return 4;
@@ -1466,8 +1494,12 @@ static void launch_stream(lbmx_lattice *L, const T *S, T *Dst, StreamArgs &a)
     a.seg_rows = stream_rows(L, D);
     a.segs = (L->Ny + a.seg_rows - 1) / a.seg_rows;
     const unsigned blocks = ((unsigned)a.strips * (unsigned)a.segs + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_stream<T, W, D, kStreamLW, P, R>), dim3(blocks), dim3(kStreamLW * D), 0,
-                       L->stream, S, Dst, a);
+    if (no_force(a))
+        hipLaunchKernelGGL((k_stream<T, W, D, kStreamLW, P, R, true>), dim3(blocks),
+                           dim3(kStreamLW * D), 0, L->stream, S, Dst, a);
+    else
+        hipLaunchKernelGGL((k_stream<T, W, D, kStreamLW, P, R>), dim3(blocks),
+                           dim3(kStreamLW * D), 0, L->stream, S, Dst, a);
 }
 
 template <typename T, bool W, int D, int P>
@@ -1479,8 +1511,12 @@ static void launch_wave(lbmx_lattice *L, const T *S, T *Dst, StreamArgs &a)
     a.segs = (L->Ny + a.seg_rows - 1) / a.seg_rows;
     const unsigned work = (unsigned)a.strips * (unsigned)a.segs;
     const unsigned blocks = ((work + kWaveWaves - 1) / kWaveWaves + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_wave<T, W, D, P>), dim3(blocks), dim3(64 * kWaveWaves), 0, L->stream,
-                       S, Dst, a);
+    if (no_force(a))
+        hipLaunchKernelGGL((k_wave<T, W, D, P, true>), dim3(blocks), dim3(64 * kWaveWaves), 0,
+                           L->stream, S, Dst, a);
+    else
+        hipLaunchKernelGGL((k_wave<T, W, D, P>), dim3(blocks), dim3(64 * kWaveWaves), 0,
+                           L->stream, S, Dst, a);
 }
 
 // stream_depth(L) gather-form steps in one launch; as pull_step2 otherwise.
@@ -1498,13 +1534,19 @@ static int pull_stream(lbmx_lattice *L)
     a.Fy_lag = s.Fy_lag;
     const T *S = (const T *)L->pop[L->pstar];
     T *D = (T *)L->pop[L->pstar ^ 1];
-    // FP64: three-row rings (40.2 KB of LDS, four workgroups per CU, 120 VGPRs) with
-    // three rows of loads in flight; FP32 storage with FP32 collision: four-row
-    // rings.  FP32 storage with FP64 collision runs the four levels in one
-    // wavefront's registers (k_wave): 160.8 vs 146.5 GLUPS at 16384^2 (FP64 102.9
-    // vs 102.8, FP32 collision at eight steps 207.6 vs 217.6, so those keep k_stream)
-    if constexpr (sizeof(T) == 8)
-        launch_stream<T, false, 4, 3, 3>(L, S, D, a);
+    // FP64: slim three-row rings (five levels: 35.1 KB of LDS, four workgroups per
+    // CU; the force-free kernel holds 80 VGPRs) with three rows of loads in flight
+    // (two: 131.3, four: 121.4 GLUPS against 132.1 at 16384^2); FP32 storage with
+    // FP32 collision: four-row rings.  FP32 storage with FP64 collision runs the
+    // four levels in one wavefront's registers (k_wave): 160.8 vs 146.5 GLUPS at
+    // 16384^2 (FP64 102.9 vs 102.8 at four steps, FP32 collision at eight steps
+    // 207.6 vs 217.6, so those keep k_stream)
+    if constexpr (sizeof(T) == 8) {
+        if (stream_depth(L) == 5)
+            launch_stream<T, false, 5, 3, 3>(L, S, D, a);
+        else
+            launch_stream<T, false, 4, 3, 3>(L, S, D, a);
+    }
     else if (g.wide)
         launch_wave<T, true, 4, 2>(L, S, D, a);
     else if (stream_depth(L) == 8)

@@ -130,11 +130,21 @@ __global__ void __launch_bounds__(256) k_collide_inplace(T *__restrict__ S, Pull
 // the same two steps.  The arithmetic is k_pull's, on the same values, so the
 // result is bit-identical to two k_pull launches.  The intermediate f*_{s+1} is
 // never stored; f*_{s+2} is all a following step or lbmx_ensure_push needs.
-template <typename T, bool W>
+// NF: no body force in this launch or in the lagged moments (Fx = Fy = Fx_lag =
+// Fy_lag = 0), so the force terms are left out at compile time (bit-identical, see
+// moments_nf).
+template <typename T, bool W, bool NF = false>
 __device__ __forceinline__ void pull_collide(T p[Q], const Geo &g, double Fx_lag, double Fy_lag,
                                              bool force, double Fx, double Fy)
 {
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (NF) {
+        Fx_lag = Fy_lag = Fx = Fy = 0.0;
+        force = false;
+    }
+    if constexpr (sizeof(T) == 8 && NF) {
+        double *f = reinterpret_cast<double *>(p);
+        collide_nf(f, moments_nf(f), g.om);
+    } else if constexpr (sizeof(T) == 8) {
         double *f = reinterpret_cast<double *>(p);
         const Moments m = moments_ref(f, Fx_lag, Fy_lag);
         collide_ref(f, m, g.om, force, Fx, Fy);
@@ -161,7 +171,7 @@ struct Pull2Args {
 constexpr int kPull2Threads = 256;
 constexpr int kPull2TX = 64;
 
-template <typename T, bool W, int TY>
+template <typename T, bool W, int TY, bool NF = false>
 __global__ void __launch_bounds__(kPull2Threads) k_pull2(const T *__restrict__ S,
                                                         T *__restrict__ D, Pull2Args a)
 {
@@ -178,7 +188,7 @@ __global__ void __launch_bounds__(kPull2Threads) k_pull2(const T *__restrict__ S
         y = (y % g.Ny + g.Ny) % g.Ny;
         T p[Q];
         gather_node(S, g, x, y, p);
-        pull_collide<T, W>(p, g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
+        pull_collide<T, W, NF>(p, g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
 #pragma unroll
         for (int k = 0; k < Q; k++) buf[k][r] = p[k];
     }
@@ -193,7 +203,7 @@ __global__ void __launch_bounds__(kPull2Threads) k_pull2(const T *__restrict__ S
 #pragma unroll
         for (int k = 0; k < Q; k++)
             p[k] = buf[k][(ty + 1 - D2Q9::cy[k]) * RX + (tx + 1 - D2Q9::cx[k])];
-        pull_collide<T, W>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
+        pull_collide<T, W, NF>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
         const size_t n = (size_t)y * g.pitch + x;
 #pragma unroll
         for (int k = 0; k < Q; k++) store_nt(&D[k * g.plane + n], p[k]);
@@ -204,7 +214,7 @@ __global__ void __launch_bounds__(kPull2Threads) k_pull2(const T *__restrict__ S
 // collides the tile and a one-node ring from it into registers, which replace the
 // LDS contents once every lane has read them, and step 3 is k_pull2's second step.
 // LDS holds 9 x 68 x (TY + 4) values.
-template <typename T, bool W, int TY>
+template <typename T, bool W, int TY, bool NF = false>
 __global__ void __launch_bounds__(kPull2Threads) k_pull3(const T *__restrict__ S,
                                                         T *__restrict__ D, Pull2Args a)
 {
@@ -223,7 +233,7 @@ __global__ void __launch_bounds__(kPull2Threads) k_pull3(const T *__restrict__ S
         y = (y % g.Ny + g.Ny) % g.Ny;
         T p[Q];
         gather_node(S, g, x, y, p);
-        pull_collide<T, W>(p, g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
+        pull_collide<T, W, NF>(p, g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
 #pragma unroll
         for (int k = 0; k < Q; k++) buf[k * AN + r] = p[k];
     }
@@ -239,7 +249,7 @@ __global__ void __launch_bounds__(kPull2Threads) k_pull3(const T *__restrict__ S
 #pragma unroll
             for (int k = 0; k < Q; k++)
                 pb[j][k] = buf[k * AN + (by + 1 - D2Q9::cy[k]) * AX + (bx + 1 - D2Q9::cx[k])];
-            pull_collide<T, W>(pb[j], g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
+            pull_collide<T, W, NF>(pb[j], g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
         }
     }
     __syncthreads();
@@ -261,7 +271,7 @@ __global__ void __launch_bounds__(kPull2Threads) k_pull3(const T *__restrict__ S
 #pragma unroll
         for (int k = 0; k < Q; k++)
             p[k] = buf[k * BN + (ty + 1 - D2Q9::cy[k]) * BX + (tx + 1 - D2Q9::cx[k])];
-        pull_collide<T, W>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
+        pull_collide<T, W, NF>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
         const size_t n = (size_t)y * g.pitch + x;
 #pragma unroll
         for (int k = 0; k < Q; k++) store_nt(&D[k * g.plane + n], p[k]);
@@ -312,21 +322,40 @@ __device__ __forceinline__ void gather_row(const T *__restrict__ S, const Geo &g
     }
 }
 
+// LDS rings of k_stream.  Level l + 1 gathers population k of its row r from row
+// r - c_y(k) of level l, so of each row a level writes, populations with c_y = -1
+// are read while it is the newest row, c_y = 0 one iteration later and c_y = +1
+// two iterations later.  With three-row rings (the new row replaces the oldest
+// between two barriers) population k therefore keeps only 2 + c_y(k) rows: 18
+// values per column instead of 27 (FP64 four levels: 26.8 instead of 40.2 KB of
+// LDS, six workgroups per CU instead of four; five levels: 35.1 KB, four per CU).
+// Four-row rings (one barrier) keep all four rows of every population: their
+// slim form, 3 + c_y(k) rows, made the FP32 eight-level kernel slower (266 vs
+// 282 GLUPS at 16384^2, 245 vs 259 at 4096^2).
 template <int D, int LW, int R>
 struct StreamShape {
     static constexpr int TX = LW - 2 * (D - 1);                  // output columns
     static constexpr int width(int l) { return LW - 2 * (l - 1); }
+    static constexpr int rows(int k) { return R == 3 ? 2 + D2Q9::cy[k] : R; }
+    static constexpr int first(int k)                            // ring rows before k's
+    {
+        int o = 0;
+        for (int j = 0; j < k; j++) o += rows(j);
+        return o;
+    }
+    static constexpr int per_col = first(Q);
     static constexpr int offset(int l)                           // LDS values before level l
     {
         int o = 0;
-        for (int m = 1; m < l; m++) o += R * Q * width(m);
+        for (int m = 1; m < l; m++) o += per_col * width(m);
         return o;
     }
     static constexpr int lds = offset(D);                        // levels 1..D-1
-    // ring slot of row r (r - rbase >= -2(D-1))
-    static __device__ __forceinline__ int slot(int r, int rbase)
+    // ring row of population k holding lattice row r (r - rbase >= -2(D-1); 12 D is
+    // a multiple of every ring length)
+    static __device__ __forceinline__ int slot(int k, int r, int rbase)
     {
-        return (int)((unsigned)(r - rbase + R * D) % (unsigned)R);
+        return first(k) + (int)((unsigned)(r - rbase + 12 * D) % (unsigned)rows(k));
     }
 };
 
@@ -335,7 +364,7 @@ struct StreamShape {
 // three-row ring the new row replaces the oldest, which the next level reads in the
 // same iteration, so the write waits for a first barrier and a second one publishes
 // it.
-template <typename T, bool W, int D, int LW, int R, int L>
+template <typename T, bool W, int D, int LW, int R, bool NF, int L>
 __device__ __forceinline__ void stream_level(T *buf, const T *__restrict__ S, T *__restrict__ Dst,
                                              const StreamArgs &a, int lane, int x0, int y0, int y1,
                                              int iters)
@@ -352,9 +381,8 @@ __device__ __forceinline__ void stream_level(T *buf, const T *__restrict__ S, T 
         if (active) {
 #pragma unroll
             for (int k = 0; k < Q; k++)
-                p[k] = in[(Sh::slot(r - D2Q9::cy[k], rbase) * Q + k) * Win + lane + 1 -
-                          D2Q9::cx[k]];
-            pull_collide<T, W>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
+                p[k] = in[Sh::slot(k, r - D2Q9::cy[k], rbase) * Win + lane + 1 - D2Q9::cx[k]];
+            pull_collide<T, W, NF>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
             if constexpr (L == D) {
                 const int x = x0 + lane;
                 if (x < g.Nx) {
@@ -370,28 +398,28 @@ __device__ __forceinline__ void stream_level(T *buf, const T *__restrict__ S, T 
         if constexpr (R == 3) __syncthreads();
         if constexpr (L < D) {
             if (active) {
-                T *out = buf + Sh::offset(L) + Sh::slot(r, rbase) * Q * Wout + lane;
+                T *out = buf + Sh::offset(L) + lane;
 #pragma unroll
-                for (int k = 0; k < Q; k++) out[k * Wout] = p[k];
+                for (int k = 0; k < Q; k++) out[Sh::slot(k, r, rbase) * Wout] = p[k];
             }
         }
         __syncthreads();
     }
 }
 
-template <typename T, bool W, int D, int LW, int R, int... Ls>
+template <typename T, bool W, int D, int LW, int R, bool NF, int... Ls>
 __device__ __forceinline__ void stream_levels(int level, T *buf, const T *__restrict__ S,
                                               T *__restrict__ Dst, const StreamArgs &a, int lane,
                                               int x0, int y0, int y1, int iters,
                                               std::integer_sequence<int, Ls...>)
 {
     ((level == Ls + 2
-          ? stream_level<T, W, D, LW, R, Ls + 2>(buf, S, Dst, a, lane, x0, y0, y1, iters)
+          ? stream_level<T, W, D, LW, R, NF, Ls + 2>(buf, S, Dst, a, lane, x0, y0, y1, iters)
           : void()),
      ...);
 }
 
-template <typename T, bool W, int D, int LW, int P, int R>
+template <typename T, bool W, int D, int LW, int P, int R, bool NF = false>
 __global__ void __launch_bounds__(LW * D) k_stream(const T *__restrict__ S, T *__restrict__ Dst,
                                                   StreamArgs a)
 {
@@ -438,19 +466,19 @@ __global__ void __launch_bounds__(LW * D) k_stream(const T *__restrict__ S, T *_
 #pragma unroll
             for (int j = 0; j < P; j++) {
                 const int t = t0 + j;
-                if (t < n1) pull_collide<T, W>(pre[j], g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
+                if (t < n1) pull_collide<T, W, NF>(pre[j], g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
                 if constexpr (R == 3) __syncthreads();
                 if (t < n1) {
-                    T *out = buf + Sh::slot(rbase + t, rbase) * Q * LW + lane;
+                    T *out = buf + lane;
 #pragma unroll
-                    for (int k = 0; k < Q; k++) out[k * LW] = pre[j][k];
+                    for (int k = 0; k < Q; k++) out[Sh::slot(k, rbase + t, rbase) * LW] = pre[j][k];
                 }
                 gather_row(S, g, row(t + P), bxm, bx, bxp, pre[j]);
                 __syncthreads();
             }
         }
     } else {
-        stream_levels<T, W, D, LW, R>(level, buf, S, Dst, a, lane, x0, y0, y1, iters,
+        stream_levels<T, W, D, LW, R, NF>(level, buf, S, Dst, a, lane, x0, y0, y1, iters,
                                       std::make_integer_sequence<int, D - 1>{});
     }
 }
@@ -520,7 +548,7 @@ __device__ __forceinline__ void wave_gather(WaveRows<T> &h, const T nw[Q], T f[Q
     h.c256[1] = nw[5];
     h.c256[2] = nw[6];
 }
-template <typename T, bool W, int D, int P>
+template <typename T, bool W, int D, int P, bool NF = false>
 __global__ void __launch_bounds__(64 * kWaveWaves) k_wave(const T *__restrict__ S,
                                                          T *__restrict__ Dst, StreamArgs a)
 {
@@ -565,12 +593,12 @@ __global__ void __launch_bounds__(64 * kWaveWaves) k_wave(const T *__restrict__ 
             for (int k = 0; k < Q; k++) nw[k] = pre[j][k];
             gather_row(S, g, row(t + P), bxm, bx, bxp, pre[j]);
             // level 1: row rbase + t
-            pull_collide<T, W>(nw, g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
+            pull_collide<T, W, NF>(nw, g, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy);
             // levels 2 .. D: row rbase + t - (l-1)
 #pragma unroll
             for (int l = 2; l <= D; l++) {
                 wave_gather(h[l - 2], nw, f);
-                pull_collide<T, W>(f, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
+                pull_collide<T, W, NF>(f, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
 #pragma unroll
                 for (int k = 0; k < Q; k++) nw[k] = f[k];
             }

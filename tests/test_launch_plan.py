@@ -8,13 +8,14 @@ from lbc_amd import native
 
 
 @pytest.mark.parametrize("Nx,Ny,prec,depth", [
-    (2048, 2048, native.F64, 4), (256, 256, native.F64, 3), (1024, 1024, native.F64, 4),
-    (512, 512, native.F64, 3), (16384, 16384, native.F64, 4),
+    (2048, 2048, native.F64, 5), (256, 256, native.F64, 3), (1024, 1024, native.F64, 4),
+    (512, 512, native.F64, 3), (16384, 16384, native.F64, 5), (1536, 1536, native.F64, 4),
+    (2048, 1800, native.F64, 5), (2048, 1600, native.F64, 4), (4096, 4096, native.F64, 5),
     (2048, 2048, native.F32_SHIFTED, 4), (1024, 1024, native.F32_SHIFTED, 4),
     (3072, 3072, native.F32_SHIFTED, 8), (768, 768, native.F32_SHIFTED, 4),
     (576, 576, native.F32_SHIFTED, 3), (640, 640, native.F32_SHIFTED, 4),
     (16384, 16384, native.F32_SHIFTED, 8), (2048, 2048, native.F32_WIDE, 4),
-    (8192, 256, native.F64, 4), (64, 100000, native.F64, 4),
+    (8192, 256, native.F64, 4), (64, 100000, native.F64, 5), (64, 30000, native.F64, 4),
     # around the streaming threshold (200 workgroups of 32 rows at four steps, 320
     # at eight, 400 for F32_WIDE)
     (576, 576, native.F64, 3), (640, 640, native.F64, 4), (704, 704, native.F64, 4),
@@ -26,7 +27,14 @@ from lbc_amd import native
 def test_auto_rule(Nx, Ny, prec, depth):
     d, cols, rows = native.launch_plan(Nx, Ny, prec)
     assert d == depth
-    if d in (4, 8):
+    if d == 5:
+        # FP64 five steps: 56-column strips, 256 rows where that still gives 4,096
+        # workgroups, else 128; only where 128-row segments give 512 workgroups
+        assert prec == native.F64 and cols == 56
+        strips = -(-Nx // cols)
+        assert rows == min(Ny, 256 if strips * -(-Ny // 256) >= 4096 else 128)
+        assert strips * -(-Ny // rows) >= 512
+    elif d in (4, 8):
         # strip of 64 - 2(D - 1) columns; rows a power of two in the depth's range
         assert cols == 64 - 2 * (d - 1)
         strips = -(-Nx // cols)
@@ -50,6 +58,9 @@ def test_auto_rule(Nx, Ny, prec, depth):
                                            else 200)
         if prec == native.F32_SHIFTED:
             assert (d == 8) == (Nx * Ny >= 6_000_000)
+        if prec == native.F64:
+            # four FP64 steps only where five would give fewer than 512 workgroups
+            assert -(-Nx // 56) * -(-Ny // 128) < 512
     else:
         # tiles of 64 columns: 16 rows with FP64 collision on FP32 storage, else
         # the tallest of 8, 4, 2 rows that gives >= 1024 tiles (2 if none does)
@@ -88,15 +99,16 @@ def test_one_round_rows():
 
 
 def test_headline_shape():
-    # 16384^2 FP64: 283 strips of 58 columns x 128 segments of 128 rows
-    assert native.launch_plan(16384, 16384, native.F64) == (4, 58, 128)
+    # 16384^2 FP64: 293 strips of 56 columns x 64 segments of 256 rows
+    assert native.launch_plan(16384, 16384, native.F64) == (5, 56, 256)
+    assert native.launch_plan(4096, 4096, native.F64) == (5, 56, 128)
     assert native.launch_plan(16384, 16384, native.F32_SHIFTED) == (8, 50, 512)
     # forced streaming keeps eight FP32 steps at any size
     assert native.launch_plan(1024, 1024, native.F32_SHIFTED, native.LAUNCH_STREAM)[0] == 8
 
 
 def test_forced_modes():
-    for prec, sd in ((native.F64, 4), (native.F32_SHIFTED, 8), (native.F32_WIDE, 4)):
+    for prec, sd in ((native.F64, 5), (native.F32_SHIFTED, 8), (native.F32_WIDE, 4)):
         assert native.launch_plan(300, 200, prec, native.LAUNCH_SINGLE) == (1, 0, 0)
         assert native.launch_plan(16384, 16384, prec, native.LAUNCH_TILES) == \
             (3, 64, 16 if prec == native.F32_WIDE else 8)

@@ -168,7 +168,7 @@ def _launch_run(sc, state, chunks, mode, precision=0):
 @pytest.mark.parametrize("sc", LAUNCH_CASES, ids=[s.name for s in LAUNCH_CASES])
 @pytest.mark.parametrize("mode", [1, 2, 3], ids=["single", "tiles", "stream"])
 def test_launch_modes_bit_identical(sc, mode):
-    """One step per launch, tiles (3 steps per launch) and row streaming (4 steps
+    """One step per launch, tiles (3 steps per launch) and row streaming (5 steps
     per launch, forced on these small lattices), over runs of 19 and 6 + 11 steps:
     every buffer and moment equals the oracle bit for bit."""
     state = sc.initial_state()
@@ -220,6 +220,30 @@ def test_fp32_four_step_stream_identical(Nx, Ny, plan):
         assert same_bits(a, b)
 
 
+@pytest.mark.parametrize("Nx,Ny,plan", [
+    # four steps: 16-, 19- and 32-row segments, the last strip and segment partial
+    (900, 900, (4, 58, 16)), (1000, 1030, (4, 58, 19)), (1280, 1280, (4, 58, 32)),
+    # five steps: 128-row segments, and a narrow lattice of one full 56-column strip
+    # and a partial 8-column one
+    (2100, 1900, (5, 56, 128)), (64, 90000, (5, 56, 128))],
+    ids=["d4_16row", "d4_19row", "d4_32row", "d5_128row", "d5_narrow"])
+@pytest.mark.parametrize("force", [False, True], ids=["noforce", "force"])
+def test_fp64_auto_stream_identical(Nx, Ny, plan, force):
+    """FP64 row streaming at the depths and segment shapes the automatic rule
+    reaches (four steps per launch on mid-size lattices, five on large ones), in the
+    force-free and the forced kernel, rounds exactly as one step per launch."""
+    from lbc_amd import native
+    F = (1e-6, -2e-6) if force else (0.0, 0.0)
+    sc = Scenario(f"tgv_f64_stream_{Nx}x{Ny}", Nx, Ny, [("periodic_x", 0), ("periodic_y", 0)],
+                  43, Fx=F[0], Fy=F[1])
+    assert native.launch_plan(sc.Nx, sc.Ny, native.F64) == plan
+    state = sc.initial_state()
+    ref = _launch_run(sc, state, (16,), native.LAUNCH_SINGLE)
+    got = _launch_run(sc, state, (16,), native.LAUNCH_AUTO)
+    for a, b in zip(got, ref):
+        assert same_bits(a, b)
+
+
 def test_launch_depth_rule():
     """LBMX_LAUNCH_AUTO streams rows once the lattice gives k_stream 200
     workgroups of 32 rows (400 with FP64 collision on FP32 storage; FP32 eight
@@ -227,7 +251,7 @@ def test_launch_depth_rule():
     lattices with walls take one step per launch unless streaming is forced, when
     the bounded multi-step launch takes them."""
     from lbc_amd import native
-    cases = [(2048, 2048, 0, 4), (256, 256, 0, 3), (1024, 1024, 0, 4), (2048, 2048, 1, 4),
+    cases = [(2048, 2048, 0, 5), (256, 256, 0, 3), (1024, 1024, 0, 4), (2048, 2048, 1, 4),
              (1024, 1024, 1, 4), (256, 256, 1, 3), (2048, 2048, 2, 4), (768, 768, 0, 4),
              (576, 576, 0, 3), (704, 704, 0, 4), (768, 768, 2, 3), (896, 896, 2, 4),
              (768, 768, 1, 4), (576, 576, 1, 3),
@@ -240,7 +264,7 @@ def test_launch_depth_rule():
         lat.set_launch(native.LAUNCH_SINGLE)
         assert lat.launch_depth() == 1
         lat.set_launch(native.LAUNCH_STREAM)
-        assert lat.launch_depth() == (8 if prec == 1 else 4)
+        assert lat.launch_depth() == (8 if prec == 1 else 5 if prec == 0 else 4)
         lat.close()
     lat = native.Lattice(64, 32, precision=0, per_x=1, per_y=0)
     mask = np.zeros((32, 64), np.int32)
