@@ -412,7 +412,7 @@ static int chan_work(lbmx_lattice *L, int NL, int TX, int per_cu, ChanArgs &a)
     return LBMX_OK;
 }
 
-template <typename T, bool W, int P, int R>
+template <typename T, bool W, int P, int R, bool NF>
 static int launch_chan(lbmx_lattice *L, T *S, T *Dst, ChanArgs &a, const ChanFusedArgs &fj)
 {
     constexpr int NL = kChanDepth + 1;
@@ -421,7 +421,7 @@ static int launch_chan(lbmx_lattice *L, T *S, T *Dst, ChanArgs &a, const ChanFus
     if (per_cu < 0) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &n, reinterpret_cast<const void *>(&k_chan<T, W, NL, kChanLW, P, R>),
+                &n, reinterpret_cast<const void *>(&k_chan<T, W, NL, kChanLW, P, R, NF>),
                 kChanLW * NL, 0) != hipSuccess ||
             n < 1)
             n = 1;
@@ -431,7 +431,7 @@ static int launch_chan(lbmx_lattice *L, T *S, T *Dst, ChanArgs &a, const ChanFus
     a.work = L->chan_work;
     const unsigned nf = (fj.a_on || fj.b_on) ? 2u * kChanDepth * kChanSplit : 0u;
     const unsigned blocks = ((unsigned)a.nwork + 7) / 8 * 8 + nf;
-    hipLaunchKernelGGL((k_chan<T, W, NL, kChanLW, P, R>), dim3(blocks), dim3(kChanLW * NL), 0,
+    hipLaunchKernelGGL((k_chan<T, W, NL, kChanLW, P, R, NF>), dim3(blocks), dim3(kChanLW * NL), 0,
                        L->stream, S, Dst, a, fj);
     return LBMX_OK;
 }
@@ -467,13 +467,19 @@ static int chan_run_t(lbmx_lattice *L)
         fj.ldf = chan_ldf_at(L, L->chan_ja_half);
         fj.part_out = L->chan_fpart + L->chan_ja_half * chan_fpart_half();
     }
+    // lattices without a body force (the cylinder channel) run the force-free
+    // instantiation
+    const bool nf = !a.force && a.Fx == 0.0 && a.Fy == 0.0 && a.Fx_lag == 0.0 && a.Fy_lag == 0.0;
     int rc;
     if constexpr (sizeof(T) == 8)
-        rc = launch_chan<T, false, 3, 3>(L, S, Dst, a, fj);
+        rc = nf ? launch_chan<T, false, 3, 3, true>(L, S, Dst, a, fj)
+                : launch_chan<T, false, 3, 3, false>(L, S, Dst, a, fj);
     else if (L->cfg.precision == LBMX_F32_WIDE)
-        rc = launch_chan<T, true, 4, 4>(L, S, Dst, a, fj);
+        rc = nf ? launch_chan<T, true, 4, 4, true>(L, S, Dst, a, fj)
+                : launch_chan<T, true, 4, 4, false>(L, S, Dst, a, fj);
     else
-        rc = launch_chan<T, false, 4, 4>(L, S, Dst, a, fj);
+        rc = nf ? launch_chan<T, false, 4, 4, true>(L, S, Dst, a, fj)
+                : launch_chan<T, false, 4, 4, false>(L, S, Dst, a, fj);
     if (rc) return rc;
     HIPCHK(hipGetLastError());
     L->chan_jb = L->chan_ja;

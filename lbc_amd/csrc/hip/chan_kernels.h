@@ -119,15 +119,26 @@ __device__ __forceinline__ Moments chan_moments(const T p[Q], double rho0, doubl
 // Collision with the node's lagged moments: for FP32 storage the moments in FP32
 // (or wide) form are recomputed from the populations unless `given` (uploaded
 // arrays, converted as k_collide_inplace does).
-template <typename T, bool W>
+// NF: no body force in the launch or in the lagged moments; the force terms are
+// left out at compile time (the moments keep the reference's sums, so the lagged
+// moments a boundary rule reads are unchanged; collide_nf equals collide_ref
+// without a force bit for bit).
+template <typename T, bool W, bool NF = false>
 __device__ __forceinline__ Moments chan_collide(T p[Q], double rho0, double om, double Fx_lag,
                                                 double Fy_lag, bool force, double Fx, double Fy,
                                                 bool given, Moments mg)
 {
+    if constexpr (NF) {
+        Fx_lag = Fy_lag = Fx = Fy = 0.0;
+        force = false;
+    }
     if constexpr (sizeof(T) == 8) {
         double *f = reinterpret_cast<double *>(p);
         const Moments m = given ? mg : moments_ref(f, Fx_lag, Fy_lag);
-        collide_ref(f, m, om, force, Fx, Fy);
+        if constexpr (NF)
+            collide_nf(f, m, om);
+        else
+            collide_ref(f, m, om, force, Fx, Fy);
         return m;
     } else if constexpr (W) {
         float *q = reinterpret_cast<float *>(p);
@@ -265,7 +276,7 @@ __device__ __forceinline__ int chan_row(const ChanArgs &a, int r, bool &in)
 
 // Level L >= 2: gathers f* of level L-1 from its ring, applies links and columns;
 // levels below NL collide into their own ring, level NL stores f_{k+D}.
-template <typename T, bool W, int NL, int LW, int P, int R, int L>
+template <typename T, bool W, int NL, int LW, int P, int R, bool NF, int L>
 __device__ __forceinline__ void chan_level(T *buf, double *mring, T *__restrict__ S,
                                            T *__restrict__ Dst, const ChanArgs &a, int lane,
                                            int x0, int y0, int y1, int iters)
@@ -355,7 +366,7 @@ __device__ __forceinline__ void chan_level(T *buf, double *mring, T *__restrict_
                 }
                 if constexpr (L < NL) {
                     const size_t n = (size_t)rw * a.pitch + Pp.xw;
-                    m = chan_collide<T, W>(p, a.rho0, chan_omega(a, code, n), a.Fx, a.Fy,
+                    m = chan_collide<T, W, NF>(p, a.rho0, chan_omega(a, code, n), a.Fx, a.Fy,
                                            a.force, a.Fx, a.Fy, false, Moments{});
                 } else if (own) {
                     const size_t o = (size_t)r * a.pitch + Pp.x;
@@ -400,13 +411,13 @@ __device__ __forceinline__ void chan_level(T *buf, double *mring, T *__restrict_
     }
 }
 
-template <typename T, bool W, int NL, int LW, int P, int R, int... Ls>
+template <typename T, bool W, int NL, int LW, int P, int R, bool NF, int... Ls>
 __device__ __forceinline__ void chan_levels(int level, T *buf, double *mring, T *__restrict__ S,
                                             T *__restrict__ Dst, const ChanArgs &a, int lane,
                                             int x0, int y0, int y1, int iters,
                                             std::integer_sequence<int, Ls...>)
 {
-    ((level == Ls + 2 ? chan_level<T, W, NL, LW, P, R, Ls + 2>(buf, mring, S, Dst, a, lane, x0,
+    ((level == Ls + 2 ? chan_level<T, W, NL, LW, P, R, NF, Ls + 2>(buf, mring, S, Dst, a, lane, x0,
                                                                 y0, y1, iters)
                       : void()),
      ...);
@@ -415,7 +426,7 @@ __device__ __forceinline__ void chan_levels(int level, T *buf, double *mring, T 
 // Plain workgroups: every node their levels compute is fluid, relaxes with the
 // uniform omega and lies away from closed edges and columns, so the levels run the
 // periodic lattice's arithmetic (as k_stream) with no per-node words or checks.
-template <typename T, bool W, int NL, int LW, int P, int R, int L>
+template <typename T, bool W, int NL, int LW, int P, int R, bool NF, int L>
 __device__ __forceinline__ void chan_plain_level(T *buf, T *__restrict__ Dst, const ChanArgs &a,
                                                  int lane, int x0, int y0, int y1, int iters)
 {
@@ -435,7 +446,7 @@ __device__ __forceinline__ void chan_plain_level(T *buf, T *__restrict__ Dst, co
                 p[k] = in[(Sh::slot(r - D2Q9::cy[k], rbase) * Q + k) * Win + lane + 1 -
                           D2Q9::cx[k]];
             if constexpr (L < NL) {
-                chan_collide<T, W>(p, a.rho0, a.om, a.Fx, a.Fy, a.force, a.Fx, a.Fy, false,
+                chan_collide<T, W, NF>(p, a.rho0, a.om, a.Fx, a.Fy, a.force, a.Fx, a.Fy, false,
                                    Moments{});
             } else if (own_x && r >= y0 && r < y1) {
                 char *row = reinterpret_cast<char *>(Dst + (size_t)r * a.pitch);
@@ -458,19 +469,19 @@ __device__ __forceinline__ void chan_plain_level(T *buf, T *__restrict__ Dst, co
     }
 }
 
-template <typename T, bool W, int NL, int LW, int P, int R, int... Ls>
+template <typename T, bool W, int NL, int LW, int P, int R, bool NF, int... Ls>
 __device__ __forceinline__ void chan_plain_levels(int level, T *buf, T *__restrict__ Dst,
                                                   const ChanArgs &a, int lane, int x0, int y0,
                                                   int y1, int iters,
                                                   std::integer_sequence<int, Ls...>)
 {
     ((level == Ls + 2
-          ? chan_plain_level<T, W, NL, LW, P, R, Ls + 2>(buf, Dst, a, lane, x0, y0, y1, iters)
+          ? chan_plain_level<T, W, NL, LW, P, R, NF, Ls + 2>(buf, Dst, a, lane, x0, y0, y1, iters)
           : void()),
      ...);
 }
 
-template <typename T, bool W, int NL, int LW, int P, int R>
+template <typename T, bool W, int NL, int LW, int P, int R, bool NF = false>
 __device__ __forceinline__ void chan_plain_first(T *buf, const T *__restrict__ S,
                                                  const ChanArgs &a, int lane, int x0, int y0,
                                                  int y1, int iters)
@@ -501,7 +512,7 @@ __device__ __forceinline__ void chan_plain_first(T *buf, const T *__restrict__ S
         for (int j = 0; j < P; j++) {
             const int t = t0 + j;
             if (t < n1)
-                chan_collide<T, W>(pre[j], a.rho0, a.om, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy,
+                chan_collide<T, W, NF>(pre[j], a.rho0, a.om, a.Fx_lag, a.Fy_lag, a.force, a.Fx, a.Fy,
                                    false, Moments{});
             if constexpr (R == 3) __syncthreads();
             if (t < n1) {
@@ -699,7 +710,7 @@ __device__ __forceinline__ void chan_force_totals(const ChanFusedArgs &a, int le
 // 2 (NL - 1) kChanSplit blocks are force workgroups: group g runs stage B of its
 // step and pair when g % kChanSplit == 0, then its share of stage A; the work
 // items follow.
-template <typename T, bool W, int NL, int LW, int P, int R>
+template <typename T, bool W, int NL, int LW, int P, int R, bool NF = false>
 __global__ void __launch_bounds__(LW * NL) __attribute__((amdgpu_waves_per_eu(4)))
 k_chan(T *__restrict__ S, T *__restrict__ Dst, ChanArgs a, ChanFusedArgs fj)
 {
@@ -731,9 +742,9 @@ k_chan(T *__restrict__ S, T *__restrict__ Dst, ChanArgs a, ChanFusedArgs fj)
 
     if (!a.lag && wk.w == 0) {
         if (level == 1)
-            chan_plain_first<T, W, NL, LW, P, R>(buf, S, a, lane, x0, y0, y1, iters);
+            chan_plain_first<T, W, NL, LW, P, R, NF>(buf, S, a, lane, x0, y0, y1, iters);
         else
-            chan_plain_levels<T, W, NL, LW, P, R>(level, buf, Dst, a, lane, x0, y0, y1, iters,
+            chan_plain_levels<T, W, NL, LW, P, R, NF>(level, buf, Dst, a, lane, x0, y0, y1, iters,
                                                   std::make_integer_sequence<int, NL - 1>{});
         return;
     }
@@ -767,7 +778,7 @@ k_chan(T *__restrict__ S, T *__restrict__ Dst, ChanArgs a, ChanFusedArgs fj)
                     const size_t n = (size_t)row(t) * a.pitch + Pp.xw;
                     Moments mg{};
                     if (a.lag) mg = Moments{a.aux->R[n], a.aux->U[n], a.aux->V[n]};
-                    m = chan_collide<T, W>(pre[j], a.rho0, chan_omega(a, npre[j].x, n), a.Fx_lag,
+                    m = chan_collide<T, W, NF>(pre[j], a.rho0, chan_omega(a, npre[j].x, n), a.Fx_lag,
                                            a.Fy_lag, a.force, a.Fx, a.Fy, a.lag, mg);
                 }
                 if constexpr (R == 3) __syncthreads();
@@ -792,7 +803,7 @@ k_chan(T *__restrict__ S, T *__restrict__ Dst, ChanArgs a, ChanFusedArgs fj)
             }
         }
     } else {
-        chan_levels<T, W, NL, LW, P, R>(level, buf, mring, S, Dst, a, lane, x0, y0, y1, iters,
+        chan_levels<T, W, NL, LW, P, R, NF>(level, buf, mring, S, Dst, a, lane, x0, y0, y1, iters,
                                         std::make_integer_sequence<int, NL - 1>{});
     }
 }
