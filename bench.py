@@ -358,7 +358,8 @@ def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True, precision
     }
 
 
-def serving_bench(rates=(10, 30, 100, 300, 1000), n=128, seed=2026, budget_gib=200):
+def serving_bench(rates=(10, 30, 100, 300, 1000), n=128, seed=2026, budget_gib=200,
+                  precision=None):
     """The serving loop: n seeded liner-duct requests (Nx in [1024, 8192], Ny in
     [64, 256], 2,000-20,000 steps, seeded impedance) arriving as a Poisson stream
     at each rate, admitted by a continuous-batching server (FP32 storage, FP64
@@ -372,7 +373,7 @@ def serving_bench(rates=(10, 30, 100, 300, 1000), n=128, seed=2026, budget_gib=2
     from lbc_amd import batch as B
     from lbc_amd import native
     from lbc_amd import serving as SV
-    prec = native.F32_WIDE
+    prec = native.F32_WIDE if precision is None else precision
     reqs = SV.draw_requests(n, seed=seed)
     # launch configuration tuned once on a representative batch
     tb = B.Batch(prec, budget=sum(B.case_bytes(r.case(), prec) for r in reqs[:16]) + (64 << 20))
@@ -444,7 +445,9 @@ def serving_bench(rates=(10, 30, 100, 300, 1000), n=128, seed=2026, budget_gib=2
         f"max {sat:.2f} solves/s ({sat / seq_rate:.2f}x); TL difference vs FP64 max "
         f"{max(tl_err):.2e} dB; mass {max(mass_err):.1e}, energy {max(energy_err):.1e} relative")
     return {
-        "workload": f"serving_liner_requests_{n}_per_rate_fp32_storage_fp64_collision",
+        "workload": f"serving_liner_requests_{n}_per_rate_"
+                    + {native.F32_WIDE: "fp32_storage_fp64_collision",
+                       native.F32_SHIFTED: "fp32_storage_fp32_collision"}[prec],
         "budget_GiB": budget, "peak_bytes": int(peak), "chunk_steps": 64,
         "tuning": {"threads": tuning[0], "tile": tuning[1], "groups_per_cu": tuning[2],
                    "order": tuning[3]},

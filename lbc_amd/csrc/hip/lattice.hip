@@ -1326,14 +1326,26 @@ static void launch_pull3(lbmx_lattice *L, const T *S, T *D, const Pull2Args &a)
 // collision.
 constexpr int kStreamLW = 64;
 
+// Output columns per strip: the LW - 2(D - 1) a workgroup's last level holds,
+// rounded down to whole 32-byte sectors of a row, so that neighbouring strips
+// never write into one sector (FP32 D = 8: 50 -> 48 columns, D = 4: 58 -> 56;
+// FP64 D = 4: 58 -> 56, D = 5: 56).  GLUPS full vs aligned: FP32 2048^2 199.5
+// vs 209.4, 16384^2 279.2 vs 279.2; FP64 1536^2 100.0 vs 105.7.
+static int stream_tx(int depth, int esize)
+{
+    const int full = kStreamLW - 2 * (depth - 1);
+    const int q = 32 / esize;
+    return full / q * q;
+}
+
 // Rows per workgroup: about 4096 workgroups over the lattice, as a power of two
 // between 32 and 128 at depth 4, and between 128 (the depth-8 pipeline takes 21
 // iterations to fill) and 512 at depth 8.  Depth 5 (FP64): 256 rows where that
 // still gives 4096 workgroups, else 128 (GLUPS 128 / 256 / 512 rows: 16384^2
 // 125.8 / 127.7-132.1 / 128.5, 8192^2 126.6 / 129.3, 4096^2 124.6 / 111.5).
-static int stream_rows_base(int Nx, int Ny, int depth)
+static int stream_rows_base(int Nx, int Ny, int depth, int esize)
 {
-    const int tx = kStreamLW - 2 * (depth - 1);
+    const int tx = stream_tx(depth, esize);
     const double strips = (double)((Nx + tx - 1) / tx);
     if (depth == 5) return std::min(strips * ((Ny + 255) / 256) >= 4096 ? 256 : 128, Ny);
     const double want = strips * Ny / 4096.0;
@@ -1345,10 +1357,10 @@ static int stream_rows_base(int Nx, int Ny, int depth)
 
 // Workgroups at the base row count (the launch thresholds below were measured
 // with it).
-static long long stream_wgs_of(int Nx, int Ny, int depth)
+static long long stream_wgs_of(int Nx, int Ny, int depth, int esize)
 {
-    const int tx = kStreamLW - 2 * (depth - 1);
-    const int rows = stream_rows_base(Nx, Ny, depth);
+    const int tx = stream_tx(depth, esize);
+    const int rows = stream_rows_base(Nx, Ny, depth, esize);
     return (long long)((Nx + tx - 1) / tx) * ((Ny + rows - 1) / rows);
 }
 
@@ -1364,11 +1376,11 @@ static long long stream_wgs_of(int Nx, int Ny, int depth)
 // grid fits one round (1024^2: 19 rows, 972 workgroups; GLUPS 16 vs 19 rows FP64
 // 74.4 vs 83.2, FP32 139.0 vs 146.8, FP32 + FP64 collision 81.5 vs 96.6).
 constexpr long long kStreamSlots = 1024;
-static int stream_rows_of(int Nx, int Ny, int depth)
+static int stream_rows_of(int Nx, int Ny, int depth, int esize)
 {
-    const int rows = stream_rows_base(Nx, Ny, depth);
-    if (depth == 4 && rows == 32 && stream_wgs_of(Nx, Ny, depth) < 768) {
-        const int tx = kStreamLW - 2 * (depth - 1);
+    const int rows = stream_rows_base(Nx, Ny, depth, esize);
+    if (depth == 4 && rows == 32 && stream_wgs_of(Nx, Ny, depth, esize) < 768) {
+        const int tx = stream_tx(depth, esize);
         const long long strips = (Nx + tx - 1) / tx;
         const long long wgs = strips * ((Ny + 15) / 16);
         if (wgs > kStreamSlots && wgs * 2 <= kStreamSlots * 3 && strips <= kStreamSlots) {
@@ -1383,7 +1395,7 @@ static int stream_rows(const lbmx_lattice *L, int depth)
 {
     static const int rows_env = getenv("LBMX_STREAM_ROWS") ? atoi(getenv("LBMX_STREAM_ROWS")) : 0;
     if (rows_env > 0) return std::min(rows_env, L->Ny);
-    return stream_rows_of(L->Nx, L->Ny, depth);
+    return stream_rows_of(L->Nx, L->Ny, depth, (int)esize_of(L->cfg.precision));
 }
 static long long stream_threshold(int precision, int depth)
 {
@@ -1395,7 +1407,7 @@ static long long stream_threshold(int precision, int depth)
 // faster one while the two FP32 population buffers mostly fit the 256 MB Infinity
 // Cache, GLUPS four vs eight steps: 1408^2 163 vs 127, 1536^2 175 vs 151, 2048^2
 // 192 vs 173; 3072^2 168 vs 205, 4096^2 153 vs 206, 8192^2 154 vs 218.
-static long long stream_wgs_of(int Nx, int Ny, int depth);
+static long long stream_wgs_of(int Nx, int Ny, int depth, int esize);
 // FP64 streams five steps per launch where the five-level grid has 512 workgroups
 // of 128 rows (from about 2048 x 1800), and always when streaming is forced; four
 // below.  GLUPS four vs five steps: 2048^2 100.4 vs 108.2, 4096^2 ~104 vs 124.6,
@@ -1404,7 +1416,7 @@ static long long stream_wgs_of(int Nx, int Ny, int depth);
 static int stream_depth_of(int Nx, int Ny, int precision, int mode)
 {
     if (precision == LBMX_F64)
-        return (mode == LBMX_LAUNCH_STREAM || stream_wgs_of(Nx, Ny, 5) >= 512) ? 5 : 4;
+        return (mode == LBMX_LAUNCH_STREAM || stream_wgs_of(Nx, Ny, 5, 8) >= 512) ? 5 : 4;
     if (precision != LBMX_F32_SHIFTED) return 4;
     if (mode == LBMX_LAUNCH_STREAM || (long long)Nx * Ny >= 6000000) return 8;
     return 4;
@@ -1426,7 +1438,8 @@ static int stream_depth(const lbmx_lattice *L)
 static bool stream_fits_of(int Nx, int Ny, int precision)
 {
     const int depth = stream_depth_of(Nx, Ny, precision, LBMX_LAUNCH_AUTO);
-    return stream_wgs_of(Nx, Ny, depth) >= stream_threshold(precision, depth);
+    return stream_wgs_of(Nx, Ny, depth, (int)esize_of(precision)) >=
+           stream_threshold(precision, depth);
 }
 static bool stream_fits(const lbmx_lattice *L)
 {
@@ -1474,8 +1487,8 @@ extern "C" int lbmx_launch_plan(int Nx, int Ny, int precision, int mode, int *st
     const int sd = stream_depth_of(Nx, Ny, precision, mode);
     if (mode == LBMX_LAUNCH_STREAM || (mode == LBMX_LAUNCH_AUTO && stream_fits_of(Nx, Ny, precision))) {
         *steps_per_launch = sd;
-        *columns = kStreamLW - 2 * (sd - 1);
-        *rows = stream_rows_of(Nx, Ny, sd);
+        *columns = stream_tx(sd, (int)esize_of(precision));
+        *rows = stream_rows_of(Nx, Ny, sd, (int)esize_of(precision));
         return LBMX_OK;
     }
     *steps_per_launch = tile_depth();
@@ -1489,8 +1502,8 @@ extern "C" int lbmx_launch_plan(int Nx, int Ny, int precision, int mode, int *st
 template <typename T, bool W, int D, int P, int R>
 static void launch_stream(lbmx_lattice *L, const T *S, T *Dst, StreamArgs &a)
 {
-    constexpr int TX = StreamShape<D, kStreamLW, R>::TX;
-    a.strips = (L->Nx + TX - 1) / TX;
+    a.tx = stream_tx(D, sizeof(T));
+    a.strips = (L->Nx + a.tx - 1) / a.tx;
     a.seg_rows = stream_rows(L, D);
     a.segs = (L->Ny + a.seg_rows - 1) / a.seg_rows;
     const unsigned blocks = ((unsigned)a.strips * (unsigned)a.segs + 7) / 8 * 8;
@@ -1505,8 +1518,8 @@ static void launch_stream(lbmx_lattice *L, const T *S, T *Dst, StreamArgs &a)
 template <typename T, bool W, int D, int P>
 static void launch_wave(lbmx_lattice *L, const T *S, T *Dst, StreamArgs &a)
 {
-    constexpr int TX = kStreamLW - 2 * (D - 1);
-    a.strips = (L->Nx + TX - 1) / TX;
+    a.tx = stream_tx(D, sizeof(T));
+    a.strips = (L->Nx + a.tx - 1) / a.tx;
     a.seg_rows = stream_rows(L, D);
     a.segs = (L->Ny + a.seg_rows - 1) / a.seg_rows;
     const unsigned work = (unsigned)a.strips * (unsigned)a.segs;

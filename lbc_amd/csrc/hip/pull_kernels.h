@@ -279,7 +279,8 @@ __global__ void __launch_bounds__(kPull2Threads) k_pull3(const T *__restrict__ S
 }
 
 // ---- D steps per launch by row streaming -------------------------------------
-// A workgroup owns a strip of TX = LW - 2(D-1) columns and a segment of rows, and
+// A workgroup owns a strip of up to LW - 2(D-1) columns (a.tx: rounded down so
+// that a strip's rows are whole 32-byte sectors) and a segment of rows, and
 // walks down the segment one row per iteration.  Time level l (1..D) is computed
 // by its own LW/64 wavefronts on a row of LW - 2(l-1) columns (the strip plus
 // the ring level l still needs); level 1 gathers from S in global memory, levels
@@ -298,6 +299,7 @@ struct StreamArgs {
     double Fx, Fy;            // body force of every step
     double Fx_lag, Fy_lag;    // force the first step's lagged moments were formed with
     int strips;               // strips per row of the lattice
+    int tx;                   // output columns per strip (at most LW - 2(D - 1))
     int seg_rows;             // output rows per workgroup
     int segs;                 // row segments
 };
@@ -385,7 +387,7 @@ __device__ __forceinline__ void stream_level(T *buf, const T *__restrict__ S, T 
             pull_collide<T, W, NF>(p, g, a.Fx, a.Fy, a.force, a.Fx, a.Fy);
             if constexpr (L == D) {
                 const int x = x0 + lane;
-                if (x < g.Nx) {
+                if (x < g.Nx && lane < a.tx) {
                     char *row = reinterpret_cast<char *>(Dst + (size_t)r * g.pitch);
                     const unsigned bx = (unsigned)x * (unsigned)sizeof(T);
 #pragma unroll
@@ -437,7 +439,7 @@ __global__ void __launch_bounds__(LW * D) k_stream(const T *__restrict__ S, T *_
     if (item >= work) return;   // whole workgroup, before any barrier
     const int y0 = (item / a.strips) * a.seg_rows;
     const int y1 = min(y0 + a.seg_rows, g.Ny);
-    const int x0 = (item % a.strips) * Sh::TX;
+    const int x0 = (item % a.strips) * a.tx;
     // every wavefront runs the same number of iterations (a multiple of P) with the
     // same barriers; iterations past a level's rows do nothing
     const int iters = ((y1 - y0) + 3 * (D - 1) + P - 1) / P * P;
@@ -552,7 +554,6 @@ template <typename T, bool W, int D, int P, bool NF = false>
 __global__ void __launch_bounds__(64 * kWaveWaves) k_wave(const T *__restrict__ S,
                                                          T *__restrict__ Dst, StreamArgs a)
 {
-    constexpr int TX = 64 - 2 * (D - 1);
     const Geo &g = a.g;
     const int lane = threadIdx.x % 64;
     // XCD-aware order: XCD i (workgroups with blockIdx % 8 == i) takes the i-th
@@ -564,13 +565,13 @@ __global__ void __launch_bounds__(64 * kWaveWaves) k_wave(const T *__restrict__ 
     if (item >= work) return;   // whole wavefront (no barrier in this kernel)
     const int y0 = (item / a.strips) * a.seg_rows;
     const int y1 = min(y0 + a.seg_rows, g.Ny);
-    const int x0 = (item % a.strips) * TX;
+    const int x0 = (item % a.strips) * a.tx;
     const int x = ((x0 - (D - 1) + lane) % g.Nx + g.Nx) % g.Nx;
     const unsigned bx = (unsigned)x * (unsigned)sizeof(T);
     const unsigned bxm = (unsigned)(x == 0 ? g.Nx - 1 : x - 1) * (unsigned)sizeof(T);
     const unsigned bxp = (unsigned)(x == g.Nx - 1 ? 0 : x + 1) * (unsigned)sizeof(T);
     const int xo = x0 + lane - (D - 1);   // output column of this lane at level D
-    const bool out_lane = lane >= D - 1 && lane <= 64 - D && xo < g.Nx;
+    const bool out_lane = lane >= D - 1 && lane < D - 1 + a.tx && xo < g.Nx;
     const int rbase = y0 - (D - 1);
     const int iters = (y1 - y0) + 2 * (D - 1);
     auto row = [&](int t) {   // periodic row of level-1 iteration t

@@ -7,6 +7,13 @@ import pytest
 from lbc_amd import native
 
 
+def strip_cols(d, prec):
+    """Output columns of a d-level strip: 64 - 2(d - 1), rounded down to whole
+    32-byte sectors of a row (FP64 4 values, FP32 storage 8)."""
+    q = 4 if prec == native.F64 else 8
+    return (64 - 2 * (d - 1)) // q * q
+
+
 @pytest.mark.parametrize("Nx,Ny,prec,depth", [
     (2048, 2048, native.F64, 5), (256, 256, native.F64, 3), (1024, 1024, native.F64, 4),
     (512, 512, native.F64, 3), (16384, 16384, native.F64, 5), (1536, 1536, native.F64, 4),
@@ -35,8 +42,9 @@ def test_auto_rule(Nx, Ny, prec, depth):
         assert rows == min(Ny, 256 if strips * -(-Ny // 256) >= 4096 else 128)
         assert strips * -(-Ny // rows) >= 512
     elif d in (4, 8):
-        # strip of 64 - 2(D - 1) columns; rows a power of two in the depth's range
-        assert cols == 64 - 2 * (d - 1)
+        # strip of 64 - 2(D - 1) columns rounded to 32-byte sectors; rows a power of
+        # two in the depth's range
+        assert cols == strip_cols(d, prec)
         strips = -(-Nx // cols)
         lo, hi = (32, 128) if d == 4 else (128, 512)
         if d == 4 and rows < 32:
@@ -60,6 +68,7 @@ def test_auto_rule(Nx, Ny, prec, depth):
             assert (d == 8) == (Nx * Ny >= 6_000_000)
         if prec == native.F64:
             # four FP64 steps only where five would give fewer than 512 workgroups
+            # (five levels: 56 columns)
             assert -(-Nx // 56) * -(-Ny // 128) < 512
     else:
         # tiles of 64 columns: 16 rows with FP64 collision on FP32 storage, else
@@ -81,11 +90,11 @@ def test_tile_rows(N, rows):
 
 
 @pytest.mark.parametrize("N,prec,rows", [
-    (768, native.F64, 16), (1024, native.F64, 19), (1280, native.F64, 32),
-    (1024, native.F32_SHIFTED, 19), (1280, native.F32_SHIFTED, 32),
-    (1024, native.F32_WIDE, 19), (1280, native.F32_WIDE, 32)])
+    (768, native.F64, 16), (1024, native.F64, 20), (1280, native.F64, 32),
+    (1024, native.F32_SHIFTED, 20), (1280, native.F32_SHIFTED, 32),
+    (1024, native.F32_WIDE, 20), (1280, native.F32_WIDE, 32)])
 def test_small_stream_rows(N, prec, rows):
-    assert native.launch_plan(N, N, prec) == (4, 58, rows)
+    assert native.launch_plan(N, N, prec) == (4, 56, rows)
 
 
 def test_one_round_rows():
@@ -102,7 +111,8 @@ def test_headline_shape():
     # 16384^2 FP64: 293 strips of 56 columns x 64 segments of 256 rows
     assert native.launch_plan(16384, 16384, native.F64) == (5, 56, 256)
     assert native.launch_plan(4096, 4096, native.F64) == (5, 56, 128)
-    assert native.launch_plan(16384, 16384, native.F32_SHIFTED) == (8, 50, 512)
+    # FP32: eight levels, 50 columns rounded to 48 (192-byte rows, six sectors)
+    assert native.launch_plan(16384, 16384, native.F32_SHIFTED) == (8, 48, 512)
     # forced streaming keeps eight FP32 steps at any size
     assert native.launch_plan(1024, 1024, native.F32_SHIFTED, native.LAUNCH_STREAM)[0] == 8
 
@@ -113,7 +123,7 @@ def test_forced_modes():
         assert native.launch_plan(16384, 16384, prec, native.LAUNCH_TILES) == \
             (3, 64, 16 if prec == native.F32_WIDE else 8)
         d, cols, rows = native.launch_plan(3, 150, prec, native.LAUNCH_STREAM)
-        assert d == sd and cols == 64 - 2 * (sd - 1) and rows <= 150
+        assert d == sd and cols == strip_cols(sd, prec) and rows <= 150
 
 
 def test_bad_arguments():

@@ -136,10 +136,11 @@ LAUNCH_CASES = GATHER_CASES + [
              Fx=2e-6, Fy=1e-6),
     Scenario("tgv_thin", 3, 150, [("periodic_x", 0), ("periodic_y", 0)], 35),
     Scenario("tgv_flat", 130, 3, [("periodic_x", 0), ("periodic_y", 0)], 36),
-    # strips that tile the width exactly (58 FP64 / 50 FP32 columns), and a strip
-    # wider than the lattice
-    Scenario("tgv_exact_strips", 116, 41, [("periodic_x", 0), ("periodic_y", 0)], 37),
-    Scenario("tgv_exact_strips32", 100, 9, [("periodic_x", 0), ("periodic_y", 0)], 38,
+    # strips that tile the width exactly (56 columns: FP64 five levels, FP32
+    # storage with FP64 collision four; 48: FP32 eight levels), and a strip wider
+    # than the lattice
+    Scenario("tgv_exact_strips", 112, 41, [("periodic_x", 0), ("periodic_y", 0)], 37),
+    Scenario("tgv_exact_strips32", 96, 9, [("periodic_x", 0), ("periodic_y", 0)], 38,
              Fx=-1e-6),
     Scenario("tgv_narrow", 17, 64, [("periodic_x", 0), ("periodic_y", 0)], 39),
     # the three-step tiles at their other heights: 4 rows (1104 tiles; the cases
@@ -196,15 +197,15 @@ def test_launch_modes_fp32_identical(sc, precision):
 
 
 @pytest.mark.parametrize("Nx,Ny,plan", [
-    # 16 strips x 57 segments of 16 rows, the last of each partial
-    (900, 900, (4, 58, 16)),
+    # 17 strips x 57 segments of 16 rows, the last of each partial
+    (900, 900, (4, 56, 16)),
     # 18 strips x 55 segments of 19 rows (16 rows would need a second round of
     # workgroups), the last of each partial
-    (1000, 1030, (4, 58, 19)),
-    # 32-row segments (920 workgroups)
-    (1280, 1280, (4, 58, 32)),
-    # a narrow lattice: one full 58-column strip and a partial 6-column one
-    (64, 90000, (4, 58, 32))], ids=["16row", "19row", "32row", "narrow"])
+    (1000, 1030, (4, 56, 19)),
+    # 32-row segments
+    (1280, 1280, (4, 56, 32)),
+    # a narrow lattice: one full 56-column strip and a partial 8-column one
+    (64, 90000, (4, 56, 32))], ids=["16row", "19row", "32row", "narrow"])
 def test_fp32_four_step_stream_identical(Nx, Ny, plan):
     """FP32 storage and collision at four steps per launch (the automatic choice
     below 6 M cells) rounds exactly as one step per launch, with a force, for the
@@ -222,7 +223,7 @@ def test_fp32_four_step_stream_identical(Nx, Ny, plan):
 
 @pytest.mark.parametrize("Nx,Ny,plan", [
     # four steps: 16-, 19- and 32-row segments, the last strip and segment partial
-    (900, 900, (4, 58, 16)), (1000, 1030, (4, 58, 19)), (1280, 1280, (4, 58, 32)),
+    (900, 900, (4, 56, 16)), (1000, 1030, (4, 56, 19)), (1280, 1280, (4, 56, 32)),
     # five steps: 128-row segments, and a narrow lattice of one full 56-column strip
     # and a partial 8-column one
     (2100, 1900, (5, 56, 128)), (64, 90000, (5, 56, 128))],
