@@ -342,7 +342,11 @@ def batched_bench(n=1024, steps=5000, sample=48, seed=2024, tune=True, precision
                        native.F64: "fp64"}[prec],
         "solves_per_s": round(n / wall, 2), "wall_s": round(wall, 3),
         "device_s": round(dev_ms / 1e3, 3), "MLUPS": round(cells * steps / wall / 1e6, 1),
-        "GB_per_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+        # per-step algorithmic bytes (72 B per cell-update); a pass of pair mode
+        # advances two steps, so the one-pass rate is half of it
+        "per_step_alg_GB_per_s": round(gbs, 1),
+        "one_pass_GB_per_s": round(gbs / (2 if pairs else 1), 1),
+        "one_pass_frac_hbm_peak": round(gbs / (2 if pairs else 1) / HBM_PEAK_GBS, 4),
         "cells": cells, "steps_per_pass": 2 if pairs else 1,
         "tuning": None if pairs else {"threads": tuning[0], "tile": tuning[1],
                                       "groups_per_cu": tuning[2], "order": tuning[3]},

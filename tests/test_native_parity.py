@@ -183,6 +183,20 @@ def test_launch_modes_bit_identical(sc, mode):
         assert same_bits(a, b)
 
 
+@pytest.mark.parametrize("sc", LAUNCH_CASES[-8:], ids=[s.name for s in LAUNCH_CASES[-8:]])
+def test_fp64_stream_four_step_remainder(sc):
+    """An FP64 run at five steps per launch whose last four steps go in one
+    four-level launch (10 = entry + 5 + 4 steps; 4 + 9 = entry + 3 tiles, then 5 +
+    4 in gather form) equals the oracle bit for bit."""
+    from lbc_amd import native
+    state = sc.initial_state()
+    for chunks in ((10,), (4, 9)):
+        ref, _ = run_oracle(sc, state, sum(chunks))
+        got = _launch_run(sc, state, chunks, native.LAUNCH_STREAM)
+        for a, b in zip(got, (ref.f, ref.f_new, ref.rho, ref.u, ref.v)):
+            assert same_bits(a, b)
+
+
 @pytest.mark.parametrize("precision", [1, 2], ids=["f32", "f32wide"])
 @pytest.mark.parametrize("sc", LAUNCH_CASES, ids=[s.name for s in LAUNCH_CASES])
 def test_launch_modes_fp32_identical(sc, precision):
