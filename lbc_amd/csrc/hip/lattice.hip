@@ -1532,8 +1532,8 @@ static void launch_wave(lbmx_lattice *L, const T *S, T *Dst, StreamArgs &a)
                            L->stream, S, Dst, a);
 }
 
-// `depth` gather-form steps in one launch (stream_depth(L), or 4 for the last
-// four steps of an FP64 run at five); as pull_step2 otherwise.
+// `depth` gather-form steps in one launch (stream_depth(L), or 4 for four to seven
+// steps left of a run at five or eight); as pull_step2 otherwise.
 template <typename T>
 static int pull_stream(lbmx_lattice *L, int depth)
 {
@@ -1563,7 +1563,7 @@ static int pull_stream(lbmx_lattice *L, int depth)
     }
     else if (g.wide)
         launch_wave<T, true, 4, 2>(L, S, D, a);
-    else if (stream_depth(L) == 8)
+    else if (depth == 8)
         launch_stream<T, false, 8, 6, 4>(L, S, D, a);
     else
         launch_stream<T, false, 4, 4, 4>(L, S, D, a);
@@ -1747,10 +1747,11 @@ extern "C" int lbmx_step(lbmx_lattice *L, int nsteps)
                 int rc = f64 ? pull_stream<double>(L, sd) : pull_stream<float>(L, sd);
                 if (rc) return rc;
             }
-            // four steps left of an FP64 run at five: one four-level launch rather
-            // than three-step tiles and a single step (16384^2: 10.5 vs 10.4 + 7.1 ms)
-            if (f64 && sd == 5 && nsteps - i == 4) {
-                int rc = pull_stream<double>(L, 4);
+            // four to seven steps left of a run at five (FP64) or eight (FP32): one
+            // four-level launch first rather than three-step tiles and single steps
+            // (FP64 16384^2: 10.5 ms for four steps vs 10.4 + 7.1 ms)
+            if (sd > 4 && nsteps - i >= 4) {
+                int rc = f64 ? pull_stream<double>(L, 4) : pull_stream<float>(L, 4);
                 if (rc) return rc;
                 i += 4;
             }

@@ -197,6 +197,20 @@ def test_fp64_stream_four_step_remainder(sc):
             assert same_bits(a, b)
 
 
+@pytest.mark.parametrize("sc", LAUNCH_CASES[-8:], ids=[s.name for s in LAUNCH_CASES[-8:]])
+def test_fp32_stream_four_step_remainder(sc):
+    """An FP32 run at eight steps per launch whose last four to seven steps start
+    with one four-level launch (13 = entry + 8 + 4; 4 + 13 = entry + 3 tiles, then
+    8 + 4 + 1) rounds exactly as one step per launch."""
+    from lbc_amd import native
+    state = sc.initial_state()
+    for chunks in ((13,), (4, 13)):
+        ref = _launch_run(sc, state, chunks, native.LAUNCH_SINGLE, native.F32_SHIFTED)
+        got = _launch_run(sc, state, chunks, native.LAUNCH_STREAM, native.F32_SHIFTED)
+        for a, b in zip(got, ref):
+            assert same_bits(a, b)
+
+
 @pytest.mark.parametrize("precision", [1, 2], ids=["f32", "f32wide"])
 @pytest.mark.parametrize("sc", LAUNCH_CASES, ids=[s.name for s in LAUNCH_CASES])
 def test_launch_modes_fp32_identical(sc, precision):
