@@ -626,6 +626,7 @@ def liner_bench(steps=20000, seed=2025):
         c = B.liner_duct(8192, 256, steps, imp, tau=0.51, amp=1e-3, x0=1024.0,
                          halfwidth=20.0, liner=(3072, 5120), sponge=192, probes=probes)
         b = B.Batch(prec, budget=B.case_bytes(c, prec) + (64 << 20))
+        b.set_pairs(True)     # two steps per pass, bit-identical to one step per launch
         i = b.add(c)
         b.synchronize()
         ms = b.time(steps)
