@@ -309,7 +309,7 @@ __device__ __forceinline__ void batch_update(const BatchLat &L, int s, int x, in
         if (force)
             collide_ref(fd, m, om, true, L.Fx, L.Fy);
         else
-            collide_ref(fd, m, om, false, 0.0, 0.0);
+            collide_nf(fd, m, om);   // collide_ref without a force, bit for bit
     } else if constexpr (W) {
         float *q = reinterpret_cast<float *>(f);
         const MomentsW m = moments_wide(q, rho0, L.Fx, L.Fy);
