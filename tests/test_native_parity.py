@@ -254,8 +254,10 @@ def test_fp32_four_step_stream_identical(Nx, Ny, plan):
     (900, 900, (4, 56, 16)), (1000, 1030, (4, 56, 19)), (1280, 1280, (4, 56, 32)),
     # five steps: 128-row segments, and a narrow lattice of one full 56-column strip
     # and a partial 8-column one
-    (2100, 1900, (5, 56, 128)), (64, 90000, (5, 56, 128))],
-    ids=["d4_16row", "d4_19row", "d4_32row", "d5_128row", "d5_narrow"])
+    (2100, 1900, (5, 56, 128)), (64, 90000, (5, 56, 128)),
+    # a flat lattice: 215 strips (the last partial) x segments of 128, 128 and 44 rows
+    (12000, 300, (5, 56, 128))],
+    ids=["d4_16row", "d4_19row", "d4_32row", "d5_128row", "d5_narrow", "d5_flat"])
 @pytest.mark.parametrize("force", [False, True], ids=["noforce", "force"])
 def test_fp64_auto_stream_identical(Nx, Ny, plan, force):
     """FP64 row streaming at the depths and segment shapes the automatic rule
