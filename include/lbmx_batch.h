@@ -214,7 +214,9 @@ typedef struct lbmx_server lbmx_server;
  * runs the batch in pair mode, lbmx_batch_set_pairs). */
 LBMX_API int lbmx_server_create(int precision, size_t budget, int chunk,
                                 const lbmx_tuning *tuning, lbmx_server **out);
-/* Queues one request (thread-safe, returns at once). */
+/* Queues one request (thread-safe, returns at once).  tl may be NULL: a request
+ * without a spectrum (a channel solve) returns its status, timing and the FP64
+ * mass and energy of its final state, with nfreq = 0. */
 LBMX_API int lbmx_server_submit(lbmx_server *s, const lbmx_case *c, const lbmx_tl_spec *tl,
                                 long long tag);
 /* Copies up to max finished results (oldest first); returns the count. */

@@ -475,9 +475,12 @@ class Server:
         check(L.lbmx_server_create(precision, int(budget), int(chunk),
                                    C.byref(t) if t is not None else None, C.byref(self.h)))
 
-    def submit(self, case: Case, tl: TlSpec, tag: int):
+    def submit(self, case: Case, tl: TlSpec | None, tag: int):
+        """Queues a request; tl None: no spectrum (a channel solve), the result
+        carries its status, timing, mass and energy only."""
         s = case.struct()
-        check(self._L.lbmx_server_submit(self.h, C.byref(s), C.byref(tl), int(tag)))
+        check(self._L.lbmx_server_submit(self.h, C.byref(s), None if tl is None else C.byref(tl),
+                                         int(tag)))
 
     def poll(self, max_results=256):
         buf = (Result * max_results)()

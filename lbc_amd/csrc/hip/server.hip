@@ -34,6 +34,7 @@ struct Request {
     lbmx_tl_spec tl;
     long long tag;
     double t_submit;
+    bool has_tl;   // false: a request without a spectrum (a channel solve)
 };
 
 struct Running {
@@ -77,13 +78,18 @@ struct lbmx_server {
     }
 };
 
-// Spectra of one finished lattice (aux stream), then TL on the host.
+// Spectra of one finished lattice (aux stream), then TL on the host; a request
+// without a spectrum returns only its totals (nfreq = 0).
 static int finish(lbmx_server *s, const Running &R, lbmx_result *out)
 {
     const BatchSlot *S = lbmx_batch_slot(s->b, R.id);
     if (!S) return lbmx_fail(LBMX_ERR_STATE, "finished lattice vanished");
-    if (int rc = lbmx_batch_spectrum_of(*S, R.r.tl, s->aux, s->spec_dev, s->spec_host, out))
-        return rc;
+    if (R.r.has_tl) {
+        if (int rc = lbmx_batch_spectrum_of(*S, R.r.tl, s->aux, s->spec_dev, s->spec_host, out))
+            return rc;
+    } else {
+        out->nfreq = 0;
+    }
     double *tot = s->tot_dev + 2 * kTotMaxBlocks;
     if (int rc = lbmx_batch_totals_of(s->b, R.id, s->tot_dev, tot, s->aux)) return rc;
     HIPCHK(hipMemcpyAsync(s->spec_host + 256, tot, 2 * sizeof(double), hipMemcpyDeviceToHost,
@@ -267,11 +273,12 @@ LBMX_API int lbmx_server_create(int precision, size_t budget, int chunk,
 LBMX_API int lbmx_server_submit(lbmx_server *s, const lbmx_case *c, const lbmx_tl_spec *tl,
                                 long long tag)
 {
-    if (!s || !c || !tl) return lbmx_fail(LBMX_ERR_ARG, "null argument");
-    if (int rc = lbmx_tl_check(c, tl)) return rc;
+    if (!s || !c) return lbmx_fail(LBMX_ERR_ARG, "null argument");
+    if (tl)
+        if (int rc = lbmx_tl_check(c, tl)) return rc;
     if (c->init == LBMX_INIT_FIELDS)
         return lbmx_fail(LBMX_ERR_ARG, "served requests start from a generated state");
-    Request r{*c, *tl, tag, s->now()};
+    Request r{*c, tl ? *tl : lbmx_tl_spec{}, tag, s->now(), tl != nullptr};
     r.c.keep_moments = 1;   // the final moments give the result's mass and energy
     {
         std::lock_guard<std::mutex> g(s->mu);

@@ -72,6 +72,37 @@ def test_same_result_alone_and_in_mix():
     assert len(overlap) >= 2, overlap
 
 
+def test_channel_requests_served_beside_liners():
+    """Channel solves (Poiseuille channels of different sizes, no spectrum) are
+    admitted and run beside liner requests: each returns status 0, no spectrum,
+    and the FP64 mass and energy of its final state equal, bit for bit, those of
+    the same case run alone in a batch in the server's step mode."""
+    chans = [B.poiseuille(256, 64, 0.8, 600), B.poiseuille(384, 96, 1.0, 900, Fx=2e-6),
+             B.poiseuille(128, 160, 0.7, 701)]
+    liners = SV.draw_requests(3, seed=5, Nx=(512, 768), Ny=(64, 96), steps=(800, 1200))
+    s = B.Server(PREC, budget=4 << 30, chunk=32)
+    for i, r in enumerate(liners):
+        s.submit(r.case(), r.tl_spec(), 10 + i)
+        time.sleep(0.002)
+        s.submit(chans[i], None, i)
+    got = wait_all(s, 6)
+    s.close()
+    for i, c in enumerate(chans):
+        r = got[i]
+        assert r.status == 0 and r.nfreq == 0 and r.steps == c.steps
+        c.keep_moments = True
+        b = B.Batch(PREC, budget=B.case_bytes(c, PREC) + (64 << 20), chunk=32)
+        b.set_pairs(True)
+        j = b.add(c)
+        b.run(c.steps + (c.steps & 1))   # pair mode advances in twos; the lattice stops at its steps
+        b.synchronize()
+        mass, energy = b.totals(j)
+        b.close()
+        assert r.mass == mass and r.energy == energy, (i, r.mass, mass, r.energy, energy)
+    for i in range(3):
+        assert got[10 + i].status == 0 and got[10 + i].nfreq == SV.NFREQ
+
+
 def test_tl_equals_spectrum_of_batch_probes():
     x = requests(3, seed=5)[1]
     s = B.Server(PREC, budget=2 << 30, chunk=16)
