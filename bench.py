@@ -762,6 +762,16 @@ def cpu_baseline(N, min_seconds=10.0, one_thread_seconds=8.0):
             "host": host}
 
 
+def _section(name, fn, *args, **kw):
+    """A secondary workload whose failure (a host-side error) is reported in its
+    place instead of ending the run before the headline line is printed."""
+    try:
+        return fn(*args, **kw)
+    except Exception as e:   # reported, not fatal: the headline number stands
+        log(f"{name} failed: {type(e).__name__}: {e}")
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -888,7 +898,7 @@ def main():
             "steps_per_launch": splw,
             "vs_fp64": round(mw / mlups, 3)}
         log(f"FP32 storage, FP64 collision {N}^2: {mw:.0f} MLUPS ({mw / mlups:.2f}x FP64)")
-        secondary["fp32_accuracy"] = fp32_accuracy()
+        secondary["fp32_accuracy"] = _section("fp32_accuracy", fp32_accuracy)
         log(f"FP32 storage accuracy: {secondary['fp32_accuracy']}")
         copy = copy_ceiling()
         hbm = traffic / (launch_ms * 1e-3) / 1e9 if traffic else None
@@ -898,19 +908,20 @@ def main():
         log(f"copy ceiling {copy:.0f} GB/s"
             + (f"; the FP64 kernel's measured traffic runs at {hbm / copy:.1%} of it" if hbm else ""))
         if args.sweep:
-            secondary["roofline_sweep"] = roofline_sweep()
-        secondary["tgv256"] = tgv256_bench()
+            secondary["roofline_sweep"] = _section("roofline_sweep", roofline_sweep)
+        secondary["tgv256"] = _section("tgv256", tgv256_bench)
         log(f"TGV 256^2: {secondary['tgv256']}")
-        secondary["cylinder"] = cylinder_bench(args.cylinder_steps)
+        secondary["cylinder"] = _section("cylinder", cylinder_bench, args.cylinder_steps)
     if args.batched > 0:
-        secondary["batched"] = batched_bench(args.batched, args.batched_steps,
-                                             args.batched_sample, pairs=not args.batched_single)
+        secondary["batched"] = _section("batched", batched_bench, args.batched,
+                                        args.batched_steps, args.batched_sample,
+                                        pairs=not args.batched_single)
     if args.serving > 0:
-        secondary["serving"] = serving_bench(n=args.serving)
+        secondary["serving"] = _section("serving", serving_bench, n=args.serving)
     if args.dropin > 0:
-        secondary["dropin"] = dropin_bench(args.dropin)
+        secondary["dropin"] = _section("dropin", dropin_bench, args.dropin)
     if args.liner > 0:
-        secondary["liner_duct"] = liner_bench(args.liner)
+        secondary["liner_duct"] = _section("liner_duct", liner_bench, args.liner)
 
     cpu = None
     if not args.no_cpu:
